@@ -1,0 +1,168 @@
+"""Benchmark: triangle-steps/s and %HBM roofline of the coupled solver step on one MI355X.
+
+Workload (N=1): the full production-scale plain method -- 981,880-triangle seeded
+Delaunay mesh with 1,474,220 edges, fractal bed, 5 land-cover classes, Horton
+infiltration, and the 118,768-node / 139,798-link / 61,317-inlet / 46-outfall
+drainage network; float32 state, FP64 ledger and clock, CFL 0.85 policy.
+A step is one solver timestep (surface flux + cell update + sources + drainage).
+Inputs are resident in HBM before the timed region; the timed region is exactly
+`--steps` steps of fh_advance_to (the call blocks until the GPU is done).
+
+Prints one JSON line (stdout).  Progress goes to stderr.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+for _p in (str(ROOT), str(ROOT / "gpu-urban-flood-cupy-jax_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+REFERENCE_CUPY_STEPS_PER_S = 758.443          # published, RTX PRO 4000 Blackwell, same mesh size
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def survey_bytes(kernel, E, C, compensated=False):
+    """Algorithmic bytes per launch (float32): each required element once per step."""
+    if kernel == "flux":
+        return 44.0 * E + 20.0 * C
+    if kernel == "update":
+        return 20.0 * E + 72.0 * C + (24.0 * C if compensated else 0.0)
+    return None
+
+
+def cpu_baseline(inputs, steps):
+    """The CPU oracle (NumPy, float32, one process) on a bounded sample of the same workload."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from helpers import to_oracle
+    from oracle import v1_parity_np as orc
+    io = to_oracle(inputs)
+    st = io[4]
+    dt = np.float32(0.25)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        rate = orc.rain_rate(io[5], i * float(dt))
+        st, _ = orc.advance_parity_step(
+            st, io[0], io[1], io[2], io[3], rainfall_rate_mps=rate, rainfall_multiplier=io[5].rainfall_multiplier,
+            horton_f0_mps=io[5].horton_f0_mps, horton_fc_mps=io[5].horton_fc_mps,
+            horton_k_per_s=io[5].horton_k_per_s, external_receiver_cell=io[5].external_receiver_cell,
+            external_q_m3ps=np.zeros(0, np.float32), dt=dt)
+    el = time.perf_counter() - t0
+    C = inputs.surface.cell_area.shape[0]
+    return C * steps / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--compensated", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--block-size", type=int, default=256)
+    ap.add_argument("--no-remap", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU only: every run here is single-GPU "
+              "(the multi-GPU sharding is a design, not a measured path). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from floodhip import synth
+    from floodhip.solver import HipParitySolver
+
+    log("generating the 981,880-triangle coupled workload")
+    t0 = time.perf_counter()
+    inp = synth.make_production_scale()
+    log(f"generated in {time.perf_counter() - t0:.1f}s")
+    C = int(inp.surface.cell_area.shape[0])
+    E = int(inp.surface.edge_left.shape[0])
+    s = HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
+                        compensated=args.compensated, use_graph=not args.no_graph,
+                        block_size=args.block_size, xcd_remap=not args.no_remap)
+    log(f"uploaded: {s.device_bytes() / 2**20:.1f} MiB resident")
+    final_t = 3600.0
+    s.advance_to(final_t, final_t, max_steps=max(args.warmup, 1), raise_on_budget=False)
+    log(f"warmup {args.warmup} steps done, t={s.clock()['time_s']:.2f}s")
+    c0 = s.clock()
+    t0 = time.perf_counter()
+    c1, ledger = s.advance_to(final_t, final_t, max_steps=args.steps, raise_on_budget=False)
+    el = time.perf_counter() - t0
+    steps = c1["global_step"] - c0["global_step"]
+    if steps != args.steps:
+        log(f"warning: only {steps} steps before t={final_t}")
+    tri_steps = C * steps / el
+    log(f"timed {steps} steps in {el:.3f}s: {1e3 * el / steps:.4f} ms/step, {tri_steps:.4e} tri-steps/s, "
+        f"t={c1['time_s']:.2f}s")
+    st = s.get_state()
+    if not (np.isfinite(st.h).all() and st.h.min() >= -1e-7):
+        log("state check FAILED")
+        sys.exit(3)
+    prof = s.profile_kernels(steps=args.profile_steps)
+    for k in prof:
+        log(f"kernel {k['kernel']:>18s}: {1e3 * k['avg_ms']:8.2f} us  x{k['launches']}")
+    by_time = sorted([k for k in prof if k["kernel"] in ("flux", "update")], key=lambda k: -k["avg_ms"])
+    dom = by_time[0]
+    ab = survey_bytes(dom["kernel"], E, C, args.compensated)
+    achieved = ab / (dom["avg_ms"] * 1e-3) / 1e9
+    kernels = {k["kernel"]: {"avg_us": round(1e3 * k["avg_ms"], 3),
+                             "algorithmic_bytes": survey_bytes(k["kernel"], E, C, args.compensated),
+                             "GBps": (round(survey_bytes(k["kernel"], E, C, args.compensated)
+                                            / (k["avg_ms"] * 1e-3) / 1e9, 1)
+                                      if survey_bytes(k["kernel"], E, C, args.compensated) else None)}
+               for k in prof}
+    cpu = None
+    if not args.no_cpu and args.cpu_steps > 0:
+        log(f"CPU baseline: oracle, {args.cpu_steps} steps of the same workload")
+        v, cel = cpu_baseline(inp, args.cpu_steps)
+        cpu = {"value": v, "unit": "triangle-steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the 981,880-triangle coupled workload "
+                         f"(NumPy float32 oracle, one process, {cel:.1f}s)"}
+        log(f"CPU baseline {v:.4e} tri-steps/s")
+    out = {
+        "metric": "triangle-steps/s + %HBM peak, 1 GPU; batched ensemble vs sequential plain",
+        "value": tri_steps,
+        "unit": "triangle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "plain, full production scale: 981,880-triangle Delaunay mesh + 118,768-node "
+                               "drainage network, float32 state, FP64 ledger/clock",
+                   "triangles": C, "edges": E, "nodes": int(inp.coupling.node_storage_area.shape[0]),
+                   "links": int(inp.pipe.link_from.shape[0]), "inlets": int(inp.coupling.inlet_cell.shape[0]),
+                   "seed": int(inp.report.get("seed", 0)), "compensated": bool(args.compensated),
+                   "hip_graph": not args.no_graph, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+        "reference_published_tri_steps_per_s": REFERENCE_CUPY_STEPS_PER_S * C,
+    }
+    s.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

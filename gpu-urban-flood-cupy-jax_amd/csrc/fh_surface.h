@@ -1,0 +1,447 @@
+// Surface step kernels: CFL/dt control, per-edge HLL flux, per-cell CSR-gather
+// update with friction, sources and velocity clamp fused.
+//
+// Semantics follow hybrid_flood/jax_solver/v1_parity.py (paths relative to
+// HF_CUPY_JAX_COMPLETED_PROJECT/reproduction/source/src/) operation by
+// operation, so float32 results round like the reference's float32 run:
+//   _hll :110-132, _hydrostatic_flux_pair :135-150, stable_timestep :153-158,
+//   surface_flux_step :161-214, apply_rainfall_horton :217-242,
+//   clamp_velocity :245-256; dt policy parity_runner.py:124-135.
+#pragma once
+#include "fh_device.h"
+
+namespace fh {
+
+// ---------------------------------------------------------------------------
+// edge encodings (set up on the host from edge_right / boundary_code / active)
+//   right >= 0 : both cells active, interior edge
+//   right == -1: ghost copies the left state (transmissive / unsupported codes)
+//   right == -2: reflective ghost (codes 0 and 2, or an inactive right cell)
+//   right == -3: outflow-only boundary (code 4): copy, inflow blocked
+// Edges whose left cell is inactive contribute exactly nothing and are dropped.
+// ---------------------------------------------------------------------------
+constexpr int kGhostCopy = -1;
+constexpr int kGhostReflect = -2;
+constexpr int kGhostOutflow = -3;
+
+// cell -> edge table entry: (edge_position << 1) | is_right_side, -1 none,
+// -2 in slot 0 marks an inactive cell.
+constexpr int kCellInactive = -2;
+
+template <typename real>
+struct EdgeFlux { real m, fuL, fvL, fuR, fvR; };
+
+// Hydrostatic reconstruction + HLL for one edge, returning flux * length for the
+// left side (m, fuL, fvL) and the right side momentum (fuR, fvR).
+template <typename real>
+__device__ __forceinline__ EdgeFlux<real> edge_flux(real h_l, real hu_l, real hv_l, real z_l,
+                                                    real h_r, real hu_r, real hv_r, real z_r,
+                                                    real nx, real ny, real len, bool block_inflow,
+                                                    const KP<real>& p) {
+    const real zero = real(0), eps = p.eps;
+    // _hydrostatic_flux_pair
+    h_l = vmax(h_l, zero);
+    h_r = vmax(h_r, zero);
+    const real z_star = vmax(z_l, z_r);
+    const real hs_l = vmax(h_l + z_l - z_star, zero);
+    const real hs_r = vmax(h_r + z_r - z_star, zero);
+    const bool wl0 = h_l > eps, wr0 = h_r > eps;
+    const real u_l0 = wl0 ? hu_l / vmax(h_l, eps) : zero;
+    const real v_l0 = wl0 ? hv_l / vmax(h_l, eps) : zero;
+    const real u_r0 = wr0 ? hu_r / vmax(h_r, eps) : zero;
+    const real v_r0 = wr0 ? hv_r / vmax(h_r, eps) : zero;
+    const real qhu_l = u_l0 * hs_l, qhv_l = v_l0 * hs_l;
+    const real qhu_r = u_r0 * hs_r, qhv_r = v_r0 * hs_r;
+    // _hll on the reconstructed states
+    const real a_l = vmax(hs_l, zero), a_r = vmax(hs_r, zero);
+    const bool wl = a_l > eps, wr = a_r > eps;
+    real fh = zero, fhu = zero, fhv = zero;
+    if (wl || wr) {
+        const real ul = wl ? qhu_l / vmax(a_l, eps) : zero;
+        const real vl = wl ? qhv_l / vmax(a_l, eps) : zero;
+        const real ur = wr ? qhu_r / vmax(a_r, eps) : zero;
+        const real vr = wr ? qhv_r / vmax(a_r, eps) : zero;
+        const real un_l = ul * nx + vl * ny;
+        const real un_r = ur * nx + vr * ny;
+        const real c_l = sqrt(p.g * a_l), c_r = sqrt(p.g * a_r);
+        const real p_l = p.half_g * (a_l * a_l), p_r = p.half_g * (a_r * a_r);
+        const real fl0 = a_l * un_l, fl1 = qhu_l * un_l + p_l * nx, fl2 = qhv_l * un_l + p_l * ny;
+        const real fr0 = a_r * un_r, fr1 = qhu_r * un_r + p_r * nx, fr2 = qhv_r * un_r + p_r * ny;
+        const real s_l = vmin(un_l - c_l, un_r - c_r);
+        const real s_r = vmax(un_l + c_l, un_r + c_r);
+        if (s_l >= zero) {
+            fh = fl0; fhu = fl1; fhv = fl2;
+        } else if (s_r <= zero) {
+            fh = fr0; fhu = fr1; fhv = fr2;
+        } else {
+            const real ds = s_r - s_l;
+            const real denom = fabs(ds) > p.tiny ? ds : real(1);
+            const real slsr = s_l * s_r;
+            fh = (s_r * fl0 - s_l * fr0 + slsr * (a_r - a_l)) / denom;
+            fhu = (s_r * fl1 - s_l * fr1 + slsr * (qhu_r - qhu_l)) / denom;
+            fhv = (s_r * fl2 - s_l * fr2 + slsr * (qhv_r - qhv_l)) / denom;
+        }
+    }
+    const real pc_l = p.half_g * (h_l * h_l - hs_l * hs_l);
+    const real pc_r = p.half_g * (h_r * h_r - hs_r * hs_r);
+    real Lh = fh + zero, Lu = fhu + pc_l * nx, Lv = fhv + pc_l * ny;
+    const real Ru = fhu + pc_r * nx, Rv = fhv + pc_r * ny;
+    if (block_inflow && Lh < zero) { Lh = zero; Lu = zero; Lv = zero; }
+    EdgeFlux<real> out;
+    out.m = Lh * len;
+    out.fuL = Lu * len;
+    out.fvL = Lv * len;
+    out.fuR = Ru * len;
+    out.fvR = Rv * len;
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// k_begin: CFL min-reduction (every `interval` steps) + dt policy + forcing lookup.
+// Wave shuffle (DPP/swizzle) -> LDS -> one grid-level pass by the last block.
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctl, KP<real> p,
+        const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc, int n_cells,
+        double* __restrict__ part, const float* __restrict__ rain_start,
+        const float* __restrict__ rain_end, const float* __restrict__ rain_rate, int n_rain,
+        const float* __restrict__ ext_times, int n_ext_times) {
+    __shared__ double lds[16 * 2];
+    const bool eval = ctl->eval_next != 0;
+    double vmin = INFINITY;
+    if (eval) {
+        real m = real(INFINITY);
+        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+            const typename V4<real>::type s = state[c];
+            const real hs = vmax(s.x, p.eps);
+            const real u = s.y / hs, v = s.z / hs;
+            real speed = sqrt(u * u + v * v);
+            speed = speed + sqrt(p.g * hs);
+            const real local = p.cfl * lc[c] / vmax(speed, p.tiny);
+            if (isfinite(local) && local < m) m = local;
+        }
+        vmin = (double)m;
+    }
+    vmin = block_min(vmin, lds);
+    double rec[1] = {vmin};
+    publish_partials<1>(part, blockIdx.x, rec);
+    if (!arrive_last(&ctl->counters[C_BEGIN], gridDim.x)) return;
+    // last block: grid-level min, then the single-thread control update
+    double gmin = INFINITY;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+        double v = __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        gmin = v < gmin ? v : gmin;
+    }
+    gmin = block_min(gmin, lds);
+    if (threadIdx.x != 0) return;
+    ctl->counters[C_BEGIN] = 0u;
+    if (eval) ctl->cfl_min = gmin;
+    const double t = ctl->t;
+    const bool active = (t < ctl->target) && (ctl->interval_step < ctl->max_steps);
+    ctl->active = active ? 1 : 0;
+    if (!active) return;
+    ctl->dt_wet = ctl->pending_wet ? ctl->dt_real : 0.0;
+    ctl->pending_wet = 1;
+    double dt;
+    double rate;
+    if (ctl->override_dt) {
+        dt = (double)(real)ctl->override_dt_value;
+        rate = ctl->override_rain;
+    } else {
+        if (p.fixed_dt > 0.0) {
+            dt = p.fixed_dt;
+        } else {
+            const double evaluated = eval ? gmin : ctl->cached_cfl;
+            const double candidate = eval ? evaluated : (double)((real)evaluated * p.reuse);
+            dt = fmax(fmin(fmin(candidate, p.max_dt), ctl->final_t - t), p.dt_floor);
+            ctl->cached_cfl = evaluated;
+            if (eval) ctl->cfl_evals += 1;
+        }
+        // parity_runner.py:96-102 -- rain interval containing t
+        int lo = 0, hi = n_rain;       // searchsorted(end, t, side="right")
+        while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)rain_end[mid] <= t) lo = mid + 1; else hi = mid; }
+        const int safe = lo < n_rain ? lo : (n_rain > 0 ? n_rain - 1 : 0);
+        const bool valid = (lo < n_rain) && (t >= (double)rain_start[safe]);
+        rate = valid ? (double)rain_rate[safe] : 0.0;
+    }
+    const real dtr = (real)dt;
+    ctl->dt_real = (double)dtr;
+    ctl->dt = (double)dtr;
+    ctl->rain_rate = (double)(real)rate;
+    // parity_runner.py:105-115 -- external inflow interpolation at t + dt/2
+    if (n_ext_times > 0) {
+        const double te = t + 0.5 * (double)dtr;
+        int lo = 0, hi = n_ext_times;
+        while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)ext_times[mid] <= te) lo = mid + 1; else hi = mid; }
+        const int upper = lo;
+        int lower = upper - 1; lower = lower < 0 ? 0 : (lower > n_ext_times - 1 ? n_ext_times - 1 : lower);
+        const int us = upper > n_ext_times - 1 ? n_ext_times - 1 : upper;
+        const double den = fmax((double)ext_times[us] - (double)ext_times[lower], 1.0e-12);
+        double frac = (te - (double)ext_times[lower]) / den;
+        frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+        ctl->ext_lower = lower;
+        ctl->ext_upper = us;
+        ctl->ext_frac = frac;
+    }
+    for (int k = 0; k < kLedgerTerms; ++k) ctl->diag[k] = 0.0;
+    ctl->t_step = t;
+    ctl->t = t + (double)dtr;
+    ctl->step += 1;
+    ctl->interval_step += 1;
+}
+
+// ---------------------------------------------------------------------------
+// k_flux: one thread per kept edge.  Interior edges [0, n_int) first, ghost
+// edges [n_int, n_edges) after, so only the tail blocks reduce boundary volume.
+// Writes the edge record rec[e] = (m, fuL, fvL, fuR), recR[e] = fvR (all * length).
+// ---------------------------------------------------------------------------
+template <typename real>
+__global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctl, KP<real> p, int n_int, int n_edges,
+        const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
+        const real* __restrict__ eny, const real* __restrict__ elen,
+        const typename V4<real>::type* __restrict__ state,
+        typename V4<real>::type* __restrict__ rec, real* __restrict__ recR, double* __restrict__ part) {
+    if (!ctl->active) return;
+    typedef typename V4<real>::type V;
+    const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    const int e = b * blockDim.x + threadIdx.x;
+    double bin = 0.0, bout = 0.0;
+    if (e < n_edges) {
+        const int l = el[e];
+        const int r = er[e];
+        const real nx = enx[e], ny = eny[e], len = elen[e];
+        const V sl = state[l];
+        real h_r, hu_r, hv_r, z_r;
+        if (r >= 0) {
+            const V sr = state[r];
+            h_r = sr.x; hu_r = sr.y; hv_r = sr.z; z_r = sr.w;
+        } else if (r == kGhostReflect) {
+            const real nm = sl.y * nx + sl.z * ny;
+            h_r = sl.x; hu_r = sl.y - real(2) * nm * nx; hv_r = sl.z - real(2) * nm * ny; z_r = sl.w;
+        } else {
+            h_r = sl.x; hu_r = sl.y; hv_r = sl.z; z_r = sl.w;
+        }
+        const EdgeFlux<real> f = edge_flux<real>(sl.x, sl.y, sl.z, sl.w, h_r, hu_r, hv_r, z_r, nx, ny, len,
+                                                 r == kGhostOutflow, p);
+        V o;
+        o.x = f.m; o.y = f.fuL; o.z = f.fvL; o.w = f.fuR;
+        rec[e] = o;
+        if (r >= 0) {
+            recR[e] = f.fvR;
+        } else {
+            const double outward = (double)(f.m * (real)ctl->dt_real);
+            bin = fmax(-outward, 0.0);
+            bout = fmax(outward, 0.0);
+        }
+    }
+    const int first_ghost_block = n_int / (int)blockDim.x;
+    if (n_edges > n_int && b >= first_ghost_block) {
+        __shared__ double lds[16 * 2];
+        double v[2] = {bin, bout};
+        block_sum<2>(v, lds);
+        publish_partials<2>(part, b - first_ghost_block, v);
+        const int nparts = (int)gridDim.x - first_ghost_block;
+        if (!arrive_last(&ctl->counters[C_FLUX], nparts)) return;
+        double s[2];
+        sum_partials<2>(part, nparts, s, lds);
+        if (threadIdx.x == 0) {
+            ctl->counters[C_FLUX] = 0u;
+            ctl->diag[L_BIN] = s[0];
+            ctl->diag[L_BOUT] = s[1];
+            ctl->diag[L_BNET] = s[0] - s[1];
+            ctl->ledger[L_BIN] += s[0];
+            ctl->ledger[L_BOUT] += s[1];
+            ctl->ledger[L_BNET] += s[0] - s[1];
+        }
+    }
+}
+
+// Error-free transformation a + b = s + err (TwoSum, Knuth).
+template <typename real>
+__device__ __forceinline__ real two_sum(real a, real b, real& err) {
+    const real s = a + b;
+    const real bb = s - a;
+    err = (a - (s - bb)) + (b - bb);
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// k_update: one thread per cell.  Gathers its (<= 3) edge records through the
+// cell->edge table (left-side entries in ascending edge order, then right-side
+// entries), applies the mass/momentum update and Manning friction
+// (v1_parity.py:203-209), rain + Horton (:217-242) and the velocity clamp
+// (:245-256), and the pending wet-time increment of the previous step
+// (:461-463).  COMP carries a per-cell TwoSum compensation of h, hu, hv.
+// ---------------------------------------------------------------------------
+template <typename real, bool COMP>
+__global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctl, KP<real> p, int n_cells,
+        const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
+        const typename V4<real>::type* __restrict__ rec, const real* __restrict__ recR,
+        typename V4<real>::type* __restrict__ state, typename V4<real>::type* __restrict__ comp,
+        real* __restrict__ wet, const real* __restrict__ area_, const real* __restrict__ n2_,
+        const real* __restrict__ rainm, const real* __restrict__ f0_, const real* __restrict__ fc_,
+        const real* __restrict__ hk_, double* __restrict__ part) {
+    if (!ctl->active) return;
+    typedef typename V4<real>::type V;
+    const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    const int c = b * blockDim.x + threadIdx.x;
+    const real dt = (real)ctl->dt_real;
+    const real dt_wet = (real)ctl->dt_wet;
+    const real rate = (real)ctl->rain_rate;
+    const real zero = real(0), eps = p.eps;
+    double rain_v = 0.0, inf_v = 0.0;
+    if (b == 0 && threadIdx.x == 0) {
+        ctl->eval_next = (p.fixed_dt <= 0.0 && ctl->override_dt == 0 && (ctl->step % p.interval) == 0) ? 1 : 0;
+    }
+    if (c < n_cells) {
+        V s = state[c];
+        // deferred wet-time increment of the previous step (h here is its final depth)
+        real wt = wet[c];
+        wt = wt + (s.x > eps ? dt_wet : zero);
+        const int e0 = ce0[c];
+        if (e0 == kCellInactive) {
+            s.x = zero; s.y = zero; s.z = zero;
+            state[c] = s;
+            wet[c] = wt;
+            if (COMP) { V z; z.x = zero; z.y = zero; z.z = zero; z.w = zero; comp[c] = z; }
+        } else {
+            const int ent[3] = {e0, ce1[c], ce2[c]};
+            real Lh = zero, Lu = zero, Lv = zero, Rh = zero, Ru = zero, Rv = zero;
+            bool anyR = false;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int en = ent[k];
+                if (en < 0) continue;
+                const int pos = en >> 1;
+                const V r = rec[pos];
+                if (en & 1) {
+                    Rh = Rh + r.x; Ru = Ru + r.w; Rv = Rv + recR[pos]; anyR = true;
+                } else {
+                    Lh = Lh + (-r.x); Lu = Lu + (-r.y); Lv = Lv + (-r.z);
+                }
+            }
+            const real dh = anyR ? Lh + Rh : Lh;
+            const real dhu = anyR ? Lu + Ru : Lu;
+            const real dhv = anyR ? Lv + Rv : Lv;
+            const real A = area_[c];
+            real h, hu, hv;
+            real ch = zero, chu = zero, chv = zero;
+            if (COMP) {
+                const V cc = comp[c];
+                h = two_sum<real>(s.x, dt * dh / A + cc.x, ch);
+                hu = two_sum<real>(s.y, dt * dhu / A + cc.y, chu);
+                hv = two_sum<real>(s.z, dt * dhv / A + cc.z, chv);
+                if (!(h > zero)) { h = zero; ch = zero; }
+            } else {
+                h = vmax(s.x + dt * dh / A, zero);
+                hu = s.y + dt * dhu / A;
+                hv = s.z + dt * dhv / A;
+            }
+            // Manning friction (semi-implicit)
+            {
+                const bool w = h > eps;
+                const real hs = vmax(h, eps);
+                const real u = hu / hs, v = hv / hs;
+                const real speed = sqrt(u * u + v * v);
+                const real damping = real(1) + dt * p.g * n2_[c] * speed / pow(hs, p.pow43);
+                if (w) { hu = hu / damping; hv = hv / damping; if (COMP) { chu = chu / damping; chv = chv / damping; } }
+                else { hu = zero; hv = zero; chu = zero; chv = zero; }
+            }
+            // rain + Horton infiltration (capacity from the wet time of the previous step)
+            const real rain_depth = vmax(rate, zero) * rainm[c] * dt;
+            real h_before, e1 = zero;
+            if (COMP) h_before = two_sum<real>(h, rain_depth, e1); else h_before = h + rain_depth;
+            const real f0 = f0_[c], fc = fc_[c];
+            const real capacity = fc + (f0 - fc) * exp(-hk_[c] * wt);
+            const real infil = vmin(h_before, vmax(capacity, zero) * dt);
+            real h2, e2 = zero;
+            if (COMP) h2 = two_sum<real>(h_before, -infil, e2); else h2 = h_before - infil;
+            const real scale = h_before > eps ? h2 / vmax(h_before, eps) : zero;
+            if (h2 > eps) { hu = hu * scale; hv = hv * scale; if (COMP) { chu = chu * scale; chv = chv * scale; } }
+            else { hu = zero; hv = zero; chu = zero; chv = zero; }
+            // velocity clamp
+            const bool w2 = h2 > eps;
+            const real speed2 = w2 ? sqrt(hu * hu + hv * hv) / vmax(h2, eps) : zero;
+            const real sc = vmin(real(1), p.max_speed / vmax(speed2, p.tiny));
+            if (w2) { hu = hu * sc; hv = hv * sc; if (COMP) { chu = chu * sc; chv = chv * sc; } }
+            else { hu = zero; hv = zero; chu = zero; chv = zero; }
+            s.x = h2; s.y = hu; s.z = hv;
+            state[c] = s;
+            wet[c] = wt;
+            if (COMP) {
+                V cc;
+                cc.x = (h2 > zero) ? (ch + e1) + e2 : zero;
+                cc.y = chu; cc.z = chv; cc.w = zero;
+                comp[c] = cc;
+            }
+            rain_v = (double)rain_depth * (double)A;
+            inf_v = (double)infil * (double)A;
+        }
+    }
+    __shared__ double lds[16 * 2];
+    double v[2] = {rain_v, inf_v};
+    block_sum<2>(v, lds);
+    publish_partials<2>(part, b, v);
+    if (!arrive_last(&ctl->counters[C_UPDATE], gridDim.x)) return;
+    double sum[2];
+    sum_partials<2>(part, gridDim.x, sum, lds);
+    if (threadIdx.x == 0) {
+        ctl->counters[C_UPDATE] = 0u;
+        ctl->diag[L_RAIN] = sum[0];
+        ctl->diag[L_INFIL] = sum[1];
+        ctl->ledger[L_RAIN] += sum[0];
+        ctl->ledger[L_INFIL] += sum[1];
+    }
+}
+
+// Applies the pending wet-time increment of the last executed step.
+template <typename real>
+__global__ void k_finalize_wet(const Ctl* __restrict__ ctl, KP<real> p, int n_cells,
+                               const typename V4<real>::type* __restrict__ state, real* __restrict__ wet) {
+    if (!ctl->pending_wet) return;
+    const real dt = (real)ctl->dt_real;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        if (state[c].x > p.eps) wet[c] = wet[c] + dt;
+    }
+}
+
+__global__ void k_clear_pending(Ctl* ctl) { ctl->pending_wet = 0; }
+
+__global__ void k_set_target(Ctl* ctl, double target, double final_t, int64_t max_steps) {
+    ctl->target = target;
+    ctl->final_t = final_t;
+    ctl->max_steps = max_steps;
+    ctl->interval_step = 0;
+    ctl->active = 0;
+}
+
+__global__ void k_set_override(Ctl* ctl, int on, double dt, double rain) {
+    ctl->override_dt = on;
+    ctl->override_dt_value = dt;
+    ctl->override_rain = rain;
+    if (on) ctl->eval_next = 0;
+}
+
+// Counts non-finite depths and node volumes (parity_runner.py:319-322 checks).
+template <typename real>
+__global__ void k_finite(Ctl* __restrict__ ctl, int n_cells, const typename V4<real>::type* __restrict__ state,
+                         int n_nodes, const real* __restrict__ vol, double* __restrict__ part) {
+    __shared__ double lds[16];
+    double bad = 0.0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells + n_nodes; c += gridDim.x * blockDim.x) {
+        const real v = c < n_cells ? state[c].x : vol[c - n_cells];
+        if (!isfinite(v)) bad += 1.0;
+    }
+    double r[1] = {bad};
+    block_sum<1>(r, lds);
+    publish_partials<1>(part, blockIdx.x, r);
+    if (!arrive_last(&ctl->counters[C_FINITE], gridDim.x)) return;
+    double s[1];
+    sum_partials<1>(part, gridDim.x, s, lds);
+    if (threadIdx.x == 0) {
+        ctl->counters[C_FINITE] = 0u;
+        ctl->nonfinite = (int)s[0];
+    }
+}
+
+}  // namespace fh

@@ -1,0 +1,175 @@
+/*
+ * floodhip -- C ABI of the MI355X per-timestep solver for the unstructured
+ * urban flood model (surface HLL shallow water + coupled 1-D drainage).
+ *
+ * One context = one scenario (or one batch of scenarios) resident on one GPU,
+ * bound to one HIP stream.  The library owns every device allocation; all
+ * pointers passed in are HOST pointers borrowed for the duration of the call
+ * (copied to the device), all pointers passed out are host buffers the caller
+ * owns.  Contexts are not thread-safe.  Arrays are float32 / int32 with the
+ * meaning of the reference's NamedTuple fields; NULL marks an optional array
+ * as absent where stated.
+ *
+ * Reference interface replaced (paths relative to
+ * HF_CUPY_JAX_COMPLETED_PROJECT/reproduction/source/src/):
+ *   hybrid_flood/operational/parity_runner.py:198-234  _advance_to_time_while
+ *   hybrid_flood/operational/parity_runner.py:118-158  _perform_numerical_step
+ *   hybrid_flood/jax_solver/v1_parity.py:430-469       advance_parity_step
+ *   hybrid_flood/jax_solver/v1_parity.py:153-158       stable_timestep
+ *   hybrid_flood/operational/parity_inputs.py:130-220  load_parity_inputs (device upload)
+ *   gurugram_flood/solver/engine.py:219-344            run_simulation hot loop
+ */
+#ifndef FLOODHIP_H
+#define FLOODHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (mapped by the Python layer to the reference's exception types,
+ * hybrid_flood/operational/errors.py:4-25 and triangular_backend.py:486-489) */
+#define FH_OK 0
+#define FH_ERR_VALUE 1          /* ValueError: bad shape / argument            */
+#define FH_ERR_SIMULATION 2     /* SimulationError: non-finite state            */
+#define FH_ERR_BUDGET 3         /* SimulationError: max_steps before target     */
+#define FH_ERR_UNAVAILABLE 4    /* BackendUnavailableError: no usable GPU       */
+#define FH_ERR_RUNTIME 5        /* HIP runtime failure                          */
+
+#define FH_LEDGER_TERMS 13      /* parity_runner.py:38-51 MassLedger field order */
+
+typedef struct fh_ctx fh_ctx;
+
+/* Solver constants.  Defaults (fh_default_params) are the reference's:
+ * v1_parity.py:14-15 (g, min depth), :245 (max speed 5), :275 (orifice
+ * 0.62*0.04), :328 (slot wave speed 30); parity_runner.py:124-135 (CFL 0.85,
+ * re-evaluate every 32 steps, x0.8 reuse, dt cap 1 s, floor 1e-3 s). */
+typedef struct {
+    double gravity;
+    double min_depth;
+    double cfl;
+    double reuse_factor;
+    double max_dt;
+    double dt_floor;
+    double max_speed;
+    double inlet_orifice_q;     /* 0.62 * 0.04 */
+    double slot_wave_speed;     /* 30 m/s      */
+    double fixed_dt;            /* > 0: every step uses exactly this dt (no CFL) */
+    int32_t cfl_interval;       /* 32 */
+    int32_t compensated;        /* 1: Kahan-compensated cell update (carried per cell) */
+    int32_t precision;          /* 32 (float state) or 64 (double state: FP64 oracle runs) */
+    int32_t use_graph;          /* 1: replay captured hipGraphs of steps_per_launch steps */
+    int32_t steps_per_launch;   /* steps per graph / per host check (default 32) */
+    int32_t block_size;         /* threads per block for the edge and cell kernels (64..1024) */
+    int32_t xcd_remap;          /* 1: map blocks so each XCD works a contiguous slice */
+    int32_t device;             /* HIP device ordinal */
+} fh_params;
+
+typedef struct {
+    double time_s;              /* FP64 simulated time                           */
+    double cached_dt_s;         /* last evaluated CFL dt                         */
+    double last_dt_s;           /* dt of the last executed step                  */
+    int64_t global_step;        /* steps executed since the clock was set        */
+    int64_t interval_steps;     /* steps executed by the last advance call        */
+    int64_t cfl_evaluations;
+} fh_clock;
+
+void fh_default_params(fh_params* p);
+const char* fh_version(void);
+int fh_device_count(void);
+
+/* Context lifetime.  fh_create returns NULL on failure (message via fh_global_error). */
+fh_ctx* fh_create(const fh_params* params);
+void fh_destroy(fh_ctx* ctx);
+const char* fh_last_error(const fh_ctx* ctx);
+const char* fh_global_error(void);
+
+/* SurfaceTopology (v1_parity.py:23-34).  active may be NULL (all active). */
+int fh_upload_surface(fh_ctx* ctx, int64_t n_cells, int64_t n_edges,
+                      const float* bed, const float* manning_n, const float* cell_area,
+                      const float* characteristic_length,
+                      const int32_t* edge_left, const int32_t* edge_right,
+                      const float* edge_nx, const float* edge_ny, const float* edge_length,
+                      const int32_t* boundary_code, const uint8_t* active);
+
+/* CouplingTopology (v1_parity.py:37-49).  n_entries = surcharge table rows (may be 0). */
+int fh_upload_coupling(fh_ctx* ctx, int64_t n_nodes, int64_t n_inlets, int64_t n_entries,
+                       const int32_t* inlet_cell, const int32_t* inlet_node,
+                       const int32_t* node_surface_cell, const float* node_invert,
+                       const float* node_storage_area, const float* node_max_depth,
+                       const float* inlet_capture_capacity, const float* inlet_surcharge_capacity,
+                       const int32_t* surcharge_cell, const int32_t* surcharge_node,
+                       const float* surcharge_weight, int32_t surcharge_weight_normalized);
+
+/* PipeTopology (v1_parity.py:52-69); outfall_* arrays have n_nodes entries. */
+int fh_upload_pipe(fh_ctx* ctx, int64_t n_links,
+                   const int32_t* link_from, const int32_t* link_to, const float* link_length,
+                   const float* link_manning, const float* link_capacity, const float* link_invert,
+                   const float* link_diameter, const float* link_minor_loss,
+                   const int32_t* link_geometry_code, const float* link_rect_width,
+                   const float* link_rect_height, const float* outfall_area,
+                   const float* outfall_coefficient, const float* outfall_tailwater,
+                   const float* outfall_capacity);
+
+/* RechargeTopology (v1_parity.py:72-77). */
+int fh_upload_recharge(fh_ctx* ctx, int64_t n_rows, const int32_t* source_cell,
+                       const float* storage_capacity, const float* capture_capacity,
+                       const float* recharge_capacity, const float* stop_depth);
+
+/* ParityForcing (parity_inputs.py:36-46).  external_q is [n_times][n_receivers] row-major. */
+int fh_set_forcing(fh_ctx* ctx, int64_t n_intervals, const float* rain_start_s,
+                   const float* rain_end_s, const float* rain_rate_mps,
+                   const float* rainfall_multiplier, const float* horton_f0_mps,
+                   const float* horton_fc_mps, const float* horton_k_per_s,
+                   int64_t n_times, const float* external_times_s,
+                   int64_t n_receivers, const int32_t* external_receiver_cell,
+                   const float* external_q_m3ps);
+
+/* ParityState (v1_parity.py:80-87).  NULL leaves a field unchanged (set) or unread (get). */
+int fh_set_state(fh_ctx* ctx, const float* h, const float* hu, const float* hv,
+                 const float* wet_time_s, const float* node_volume, const float* link_flow,
+                 const float* recharge_storage);
+int fh_get_state(fh_ctx* ctx, float* h, float* hu, float* hv, float* wet_time_s,
+                 float* node_volume, float* link_flow, float* recharge_storage);
+/* FP64 views of the cell state (meaningful when precision == 64). */
+int fh_get_state_f64(fh_ctx* ctx, double* h, double* hu, double* hv);
+
+/* Clock and ledger (parity_runner.py:54-61 AdvanceCarry).  The ledger is 13
+ * FP64 terms in MassLedger order, kept on the device. */
+int fh_set_clock(fh_ctx* ctx, double time_s, double cached_dt_s, int64_t global_step);
+int fh_get_clock(fh_ctx* ctx, fh_clock* out);
+int fh_set_ledger(fh_ctx* ctx, const double* ledger13);
+int fh_get_ledger(fh_ctx* ctx, double* ledger13);
+
+/* _advance_to_time_while: step while time < target_s and fewer than max_steps
+ * steps were taken in this call; dt = max(min(cfl_or_reuse, max_dt, final_s - t), floor).
+ * Blocks until done.  Returns FH_ERR_BUDGET when the budget ran out first and
+ * FH_ERR_SIMULATION when depth or node volume became non-finite. */
+int fh_advance_to(fh_ctx* ctx, double target_s, double final_s, int64_t max_steps,
+                  fh_clock* clock_out, double* ledger_out);
+
+/* advance_parity_step with an explicit dt and rain rate (v1_parity.py:430-469):
+ * one step, external inflow q given per receiver (may be NULL = 0).  diag_out gets
+ * the 13 step diagnostics (StepDiagnostics order).  The clock advances by dt. */
+int fh_step(fh_ctx* ctx, double dt, double rain_rate_mps, const float* external_q_m3ps,
+            double* diag_out);
+
+/* stable_timestep (v1_parity.py:153-158) on the resident state. */
+int fh_stable_timestep(fh_ctx* ctx, double* dt_out);
+
+/* Measurement: runs `steps` steps of the plain step sequence (no graph), timing
+ * each kernel with HIP events on the context stream.  names/ms/bytes are
+ * arrays of length cap; returns the number of kernels reported in *count.
+ * bytes = algorithmic bytes per launch of that kernel. */
+int fh_profile_kernels(fh_ctx* ctx, int32_t steps, int32_t cap, int32_t* count,
+                       char* names /* cap x 32 chars */, double* avg_ms, double* bytes,
+                       int64_t* launches);
+
+/* Size of the resident device footprint in bytes. */
+int64_t fh_device_bytes(const fh_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLOODHIP_H */

@@ -1,0 +1,132 @@
+"""Shared test helpers: scenario builders and oracle adapters."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from floodhip import synth, types
+from floodhip.topology import build_surface_topology, cell_centroids
+from oracle import v1_parity_np as orc
+
+
+def to_oracle(inputs, dtype=np.float32):
+    """Convert a LoadedParityInputs (product types) to oracle NamedTuples in `dtype`."""
+    s = orc.cast_tuple(orc.SurfaceTopology(*inputs.surface), dtype)
+    c = orc.cast_tuple(orc.CouplingTopology(*inputs.coupling), dtype)
+    p = orc.cast_tuple(orc.PipeTopology(*inputs.pipe), dtype)
+    r = orc.cast_tuple(orc.RechargeTopology(*inputs.recharge), dtype)
+    st = orc.cast_tuple(orc.ParityState(*inputs.initial_state), dtype)
+    f = orc.cast_tuple(orc.ParityForcing(*inputs.forcing), dtype)
+    return s, c, p, r, st, f
+
+
+def oracle_step(inputs_o, state, dt, rain=0.0, q=None, dtype=np.float32):
+    s, c, p, r, _, f = inputs_o
+    if q is None:
+        q = np.zeros(np.asarray(f.external_receiver_cell).shape[0], dtype)
+    return orc.advance_parity_step(
+        state, s, c, p, r, rainfall_rate_mps=rain, rainfall_multiplier=f.rainfall_multiplier,
+        horton_f0_mps=f.horton_f0_mps, horton_fc_mps=f.horton_fc_mps, horton_k_per_s=f.horton_k_per_s,
+        external_receiver_cell=f.external_receiver_cell, external_q_m3ps=q, dt=dt, dtype=dtype)
+
+
+def basin(n=12, extent=120.0, code="closed", bed=None, h0=None, manning=0.0, active=None, seed=0):
+    """Split-square basin with uniform boundary code; returns LoadedParityInputs."""
+    nodes, tri = synth.split_square_grid(n, extent)
+    cen = cell_centroids(nodes, tri)
+    z = np.zeros(tri.shape[0], np.float32) if bed is None else np.asarray(bed(cen), np.float32)
+    surface, tri = build_surface_topology(nodes, tri, z, manning, active_mask=active, boundary_mode=code)
+    C = tri.shape[0]
+    forcing = synth.rain_forcing([0.0], 1.0e9, C)
+    h = np.zeros(C, np.float32) if h0 is None else np.asarray(h0(cen), np.float32)
+    state = types.ParityState(h, np.zeros(C, np.float32), np.zeros(C, np.float32), np.zeros(C, np.float32),
+                              np.zeros(0, np.float32), np.zeros(0, np.float32), np.zeros(0, np.float32))
+    return types.LoadedParityInputs(surface, types.empty_coupling(), types.empty_pipe(),
+                                    types.empty_recharge(), state, forcing,
+                                    {"nodes": nodes, "triangles": tri, "centroids": cen}, {})
+
+
+def small_network_scenario(seed=7, n=24, extent=600.0, n_nodes=40, n_links=52, n_inlets=30, n_outfalls=2,
+                           with_entries=True, with_recharge=True, with_external=True, h_init=0.05):
+    """A small fully coupled scenario exercising every drainage path."""
+    rng = np.random.default_rng(seed)
+    nodes, tri = synth.jittered_delaunay(n, n, extent, extent, rng)
+    cen = cell_centroids(nodes, tri)
+    bed = (8.0 * synth.value_noise(cen[:, 0], cen[:, 1], extent, rng, octaves=3)).astype(np.float32)
+    C = tri.shape[0]
+    active = rng.random(C) > 0.05
+    surface, tri = build_surface_topology(nodes, tri, bed, rng.uniform(0.02, 0.06, C).astype(np.float32),
+                                          active_mask=active, boundary_mode="outflow")
+    # make a few boundary edges reflective and transmissive to exercise all ghost kinds
+    code = surface.boundary_code.copy()
+    bidx = np.flatnonzero(surface.edge_right < 0)
+    code[bidx[::5]] = 0
+    code[bidx[1::7]] = 1
+    code[bidx[2::11]] = 2
+    surface = surface._replace(boundary_code=code)
+    coupling, pipe, v0 = synth.pipe_network(rng, cen, bed, n_nodes, n_links, n_inlets, n_outfalls, extent)
+    # shallow pipes so surcharge, overflow and capture all happen
+    coupling = coupling._replace(node_max_depth=np.minimum(coupling.node_max_depth, 1.2).astype(np.float32),
+                                 node_invert=(bed[coupling.node_surface_cell] - 1.2).astype(np.float32))
+    v0 = (coupling.node_storage_area * np.float32(0.9) * coupling.node_max_depth).astype(np.float32)
+    if with_entries:
+        S = 25
+        sc = rng.integers(0, C, S).astype(np.int32)
+        sn = rng.integers(0, n_nodes // 2, S).astype(np.int32)
+        sn[3] = -1  # an invalid row
+        coupling = coupling._replace(surcharge_cell=sc, surcharge_node=sn,
+                                     surcharge_weight=rng.uniform(0.0, 2.0, S).astype(np.float32))
+    nsc = coupling.node_surface_cell.copy()
+    nsc[-1] = -1   # a node without a surface cell: its overflow is unreturned
+    coupling = coupling._replace(node_surface_cell=nsc)
+    if with_recharge:
+        R = 12
+        rc = rng.integers(0, C, R).astype(np.int32)
+        rc[1] = rc[0]
+        recharge = types.RechargeTopology(rc, rng.uniform(5, 50, R).astype(np.float32),
+                                          rng.uniform(0.01, 0.1, R).astype(np.float32),
+                                          rng.uniform(0.001, 0.01, R).astype(np.float32),
+                                          rng.uniform(0.0, 0.03, R).astype(np.float32))
+    else:
+        recharge = types.empty_recharge()
+    f0 = rng.uniform(1e-6, 2e-6, C).astype(np.float32)
+    fc = (f0 * 0.3).astype(np.float32)
+    k = np.full(C, 4.0 / 3600.0, np.float32)
+    forcing = synth.rain_forcing([30.0, 80.0, 10.0], 60.0, C, multiplier=rng.uniform(0.8, 1.2, C),
+                                 f0=f0, fc=fc, k=k)
+    if with_external:
+        rcv = rng.integers(0, C, 3).astype(np.int32)
+        rcv[2] = rcv[0]
+        times = np.array([0.0, 30.0, 90.0, 400.0], np.float32)
+        q = rng.uniform(0.0, 0.5, (4, 3)).astype(np.float32)
+        q[1, 1] = -0.2
+        forcing = forcing._replace(external_times_s=times, external_receiver_cell=rcv, external_q_m3ps=q)
+    h0 = np.where(surface.active, h_init * rng.random(C), 0).astype(np.float32)
+    state = types.ParityState(h0, np.zeros(C, np.float32), np.zeros(C, np.float32),
+                              np.zeros(C, np.float32), v0, np.zeros(pipe.link_from.shape[0], np.float32),
+                              np.zeros(recharge.source_cell.shape[0], np.float32))
+    return types.LoadedParityInputs(surface, coupling, pipe, recharge, state, forcing,
+                                    {"nodes": nodes, "triangles": tri, "centroids": cen}, {})
+
+
+def stoker_depth(x, t, h_l, h_r, x0, g=9.80665):
+    """Stoker's analytic wet-bed dam-break depth (1-D), solved for the shock state."""
+    from scipy.optimize import brentq
+    c_l, c_r = np.sqrt(g * h_l), np.sqrt(g * h_r)
+
+    def f(hm):  # continuity of velocity across rarefaction and shock
+        um_raref = 2.0 * (c_l - np.sqrt(g * hm))
+        s = np.sqrt(0.5 * g * hm * (hm + h_r) / h_r) if h_r > 0 else 0.0  # shock speed (u_r = 0)
+        um_shock = s * (1.0 - h_r / hm)
+        return um_raref - um_shock
+
+    hm = brentq(f, h_r * (1 + 1e-12), h_l)
+    um = 2.0 * (c_l - np.sqrt(g * hm))
+    s = np.sqrt(0.5 * g * hm * (hm + h_r) / h_r)
+    xi = (x - x0) / max(t, 1e-12)
+    h = np.where(xi <= -c_l, h_l, 0.0)
+    raref = (xi > -c_l) & (xi <= um - np.sqrt(g * hm))
+    h = np.where(raref, (2 * c_l - xi) ** 2 / (9 * g), h)
+    h = np.where((xi > um - np.sqrt(g * hm)) & (xi <= s), hm, h)
+    h = np.where(xi > s, h_r, h)
+    return h

@@ -1,0 +1,102 @@
+"""GPU parity of the coupled step (surface + external inflow + inlets + links +
+outfalls + overflow return + recharge) against the CPU oracle.
+
+Tolerances (float32): per step |dh| <= 1e-5 m; node volume within 1e-4 relative
+(of max(V, 1 m^3)); link flow within 1e-4 relative (of max(|Q|, 1e-3 m^3/s));
+ledger terms within 1e-4 relative.  The drainage sums run in the reference's
+scatter order, so differences come only from device-library transcendentals.
+"""
+
+import numpy as np
+import pytest
+
+from floodhip import synth
+from floodhip.solver import HipParitySolver
+from helpers import oracle_step, small_network_scenario, to_oracle
+from oracle import v1_parity_np as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(inp, **kw):
+    return HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing,
+                           inp.initial_state, **kw)
+
+
+def _compare(got, ref, diag=None, rdiag=None):
+    assert np.max(np.abs(got.h - ref.h)) <= 1e-5
+    vs = np.maximum(np.abs(ref.node_volume), 1.0)
+    assert np.max(np.abs(got.node_volume - ref.node_volume) / vs) <= 1e-4
+    qs = np.maximum(np.abs(ref.link_flow), 1e-3)
+    assert np.max(np.abs(got.link_flow - ref.link_flow) / qs) <= 1e-4
+    assert np.max(np.abs(got.recharge_storage - ref.recharge_storage)) <= 1e-4 * max(1.0, float(
+        np.max(np.abs(ref.recharge_storage), initial=0.0)))
+    if diag is not None:
+        for k in orc.LEDGER_FIELDS:
+            r = rdiag[k]
+            assert abs(getattr(diag, k) - r) <= 1e-4 * max(abs(r), 1e-3), (k, getattr(diag, k), r)
+
+
+@pytest.mark.parametrize("entries,normalized", [(True, False), (True, True), (False, False)])
+def test_coupled_steps_match_oracle(entries, normalized):
+    inp = small_network_scenario(with_entries=entries)
+    if entries and normalized:
+        w = inp.coupling.surcharge_weight
+        inp = inp._replace(coupling=inp.coupling._replace(surcharge_weight=(w / w.sum()).astype(np.float32),
+                                                          surcharge_weight_normalized=True))
+    io = to_oracle(inp)
+    st = io[4]
+    t = 0.0
+    with _solver(inp) as s:
+        for i in range(25):
+            dt = 0.3 if i % 2 else 0.45
+            rate = orc.rain_rate(io[5], t)
+            q = orc.external_q(io[5], t + 0.5 * dt)
+            st, rdiag = oracle_step(io, st, dt, rain=rate, q=q)
+            diag = s.step(dt, rate, q)
+            t += dt
+            got = s.get_state()
+            _compare(got, st, diag, rdiag)
+
+
+def test_coupled_policy_advance_matches_oracle():
+    inp = small_network_scenario(seed=11, h_init=0.2)
+    io = to_oracle(inp)
+    st, clock, led = orc.advance_to_time(io[4], orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger(), 50.0, 120.0,
+                                         *io[:4], io[5], max_steps=10_000)
+    with _solver(inp) as s:
+        c, ledger = s.advance_to(50.0, 120.0, max_steps=10_000)
+        got = s.get_state()
+    assert c["global_step"] == clock.global_step
+    assert np.max(np.abs(got.h - st.h)) <= 1e-4
+    for k in ("rainfall_input", "surface_capture", "outfall_discharge", "external_boundary_inflow",
+              "recharge_capture"):
+        r = led[k]
+        assert abs(getattr(ledger, k) - r) <= 1e-3 * max(abs(r), 1e-3), (k, getattr(ledger, k), r)
+
+
+def test_catchment_with_network_fixed_dt_200_steps():
+    inp = synth.make_catchment(20_000, seed=synth.BASE_SEED + 40, rain_total_mm=120.0, rain_hours=1,
+                               interval_s=3600.0, rain_mm_h=[120.0])
+    io = to_oracle(inp)
+    st = io[4]
+    dt = 0.5
+    for i in range(200):
+        st, _ = oracle_step(io, st, dt, rain=orc.rain_rate(io[5], i * dt))
+    with _solver(inp, fixed_dt=dt) as s:
+        s.advance_to(200 * dt, 1e9, max_steps=200)
+        got = s.get_state()
+    assert np.max(np.abs(got.h - st.h)) <= 1e-4
+    vs = np.maximum(np.abs(st.node_volume), 1.0)
+    assert np.max(np.abs(got.node_volume - st.node_volume) / vs) <= 1e-3
+
+
+def test_coupled_bitwise_repeatable():
+    inp = small_network_scenario(seed=3)
+    outs = []
+    for _ in range(2):
+        with _solver(inp, steps_per_launch=8) as s:
+            s.advance_to(40.0, 100.0)
+            st = s.get_state()
+            outs.append(b"".join(np.asarray(x).tobytes() for x in st) + np.asarray(tuple(s.ledger())).tobytes())
+    assert outs[0] == outs[1]
