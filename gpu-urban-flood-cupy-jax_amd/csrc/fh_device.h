@@ -63,6 +63,16 @@ struct Ctl {
     unsigned counters[C_COUNT];
 };
 
+// Block -> member mapping of one index space (cells, edges, nodes, ...).  Each
+// member's items start on a tile boundary (tile = blockDim.x * items per
+// thread), so a block never straddles two members.
+struct Seg {
+    const int* blk_member;   // [blocks]   member of each (logical) block
+    const int* mem_blk0;     // [members]  first block of the member
+    const int* mem_nblk;     // [members]  number of blocks of the member
+    const int* mem_end;      // [members]  exclusive end of the member's real items
+};
+
 template <typename real>
 struct KP {                 // kernel constants, pre-rounded to the state precision
     real g, eps, half_g, two_g, cfl, max_speed, tiny, capture_q, pow43, reuse, slot_coef;

@@ -1,6 +1,11 @@
 // Surface step kernels: CFL/dt control, per-edge HLL flux, per-cell CSR-gather
 // update with friction, sources and velocity clamp fused.
 //
+// Every kernel serves a batch of members (independent scenarios, each with its
+// own clock, dt, forcing and ledger): the index spaces are concatenated member
+// by member on tile boundaries and a Seg table maps each block to its member.
+// A single scenario is the one-member case.
+//
 // Semantics follow hybrid_flood/jax_solver/v1_parity.py (paths relative to
 // HF_CUPY_JAX_COMPLETED_PROJECT/reproduction/source/src/) operation by
 // operation, so float32 results round like the reference's float32 run:
@@ -25,7 +30,7 @@ constexpr int kGhostReflect = -2;
 constexpr int kGhostOutflow = -3;
 
 // cell -> edge table entry: (edge_position << 1) | is_right_side, -1 none,
-// -2 in slot 0 marks an inactive cell.
+// -2 in slot 0 marks an inactive (or padding) cell.
 constexpr int kCellInactive = -2;
 
 template <typename real>
@@ -96,40 +101,55 @@ __device__ __forceinline__ EdgeFlux<real> edge_flux(real h_l, real hu_l, real hv
     return out;
 }
 
+// Per-member tables of the control kernel.
+struct BeginTables {
+    Seg seg;                 // k_begin blocks of each member
+    const int* cell0;        // [M] first cell of the member
+    const int* cell1;        // [M] end of the member's real cells
+    const int* rain_off;     // [M] offset of the member's rain intervals
+    const int* rain_n;       // [M] number of rain intervals
+    const int* ext_off;      // [M] offset of the member's external time table
+    const int* ext_n;        // [M] number of external times (0: no external inflow)
+};
+
 // ---------------------------------------------------------------------------
-// k_begin: CFL min-reduction (every `interval` steps) + dt policy + forcing lookup.
-// Wave shuffle (DPP/swizzle) -> LDS -> one grid-level pass by the last block.
+// k_begin: per-member CFL min-reduction (every `interval` steps) + dt policy +
+// forcing lookup.  Wave shuffle -> LDS -> one grid-level pass by the member's
+// last-arriving block.
 // ---------------------------------------------------------------------------
 template <typename real>
-__global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctl, KP<real> p,
-        const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc, int n_cells,
-        double* __restrict__ part, const float* __restrict__ rain_start,
-        const float* __restrict__ rain_end, const float* __restrict__ rain_rate, int n_rain,
-        const float* __restrict__ ext_times, int n_ext_times) {
+__global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> p,
+        const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc, BeginTables bt,
+        double* __restrict__ part, const float* __restrict__ rain_start, const float* __restrict__ rain_end,
+        const float* __restrict__ rain_rate, const float* __restrict__ ext_times) {
     __shared__ double lds[16 * 2];
+    const int b = blockIdx.x;
+    const int m = bt.seg.blk_member[b];
+    Ctl* ctl = ctls + m;
+    const int blk0 = bt.seg.mem_blk0[m], nblk = bt.seg.mem_nblk[m];
     const bool eval = ctl->eval_next != 0;
-    double vmin = INFINITY;
+    double vmin_ = INFINITY;
     if (eval) {
-        real m = real(INFINITY);
-        for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
+        real mn = real(INFINITY);
+        const int c1 = bt.cell1[m];
+        for (int c = bt.cell0[m] + (b - blk0) * blockDim.x + threadIdx.x; c < c1; c += nblk * blockDim.x) {
             const typename V4<real>::type s = state[c];
             const real hs = vmax(s.x, p.eps);
             const real u = s.y / hs, v = s.z / hs;
             real speed = sqrt(u * u + v * v);
             speed = speed + sqrt(p.g * hs);
             const real local = p.cfl * lc[c] / vmax(speed, p.tiny);
-            if (isfinite(local) && local < m) m = local;
+            if (isfinite(local) && local < mn) mn = local;
         }
-        vmin = (double)m;
+        vmin_ = (double)mn;
     }
-    vmin = block_min(vmin, lds);
-    double rec[1] = {vmin};
-    publish_partials<1>(part, blockIdx.x, rec);
-    if (!arrive_last(&ctl->counters[C_BEGIN], gridDim.x)) return;
-    // last block: grid-level min, then the single-thread control update
+    vmin_ = block_min(vmin_, lds);
+    double rec[1] = {vmin_};
+    publish_partials<1>(part, b, rec);
+    if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
     double gmin = INFINITY;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-        double v = __hip_atomic_load(part + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+        double v = __hip_atomic_load(part + blk0 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         gmin = v < gmin ? v : gmin;
     }
     gmin = block_min(gmin, lds);
@@ -158,26 +178,31 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctl, KP<real> p
             if (eval) ctl->cfl_evals += 1;
         }
         // parity_runner.py:96-102 -- rain interval containing t
+        const int n_rain = bt.rain_n[m];
+        const float* rs = rain_start + bt.rain_off[m];
+        const float* re = rain_end + bt.rain_off[m];
         int lo = 0, hi = n_rain;       // searchsorted(end, t, side="right")
-        while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)rain_end[mid] <= t) lo = mid + 1; else hi = mid; }
+        while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)re[mid] <= t) lo = mid + 1; else hi = mid; }
         const int safe = lo < n_rain ? lo : (n_rain > 0 ? n_rain - 1 : 0);
-        const bool valid = (lo < n_rain) && (t >= (double)rain_start[safe]);
-        rate = valid ? (double)rain_rate[safe] : 0.0;
+        const bool valid = (lo < n_rain) && (t >= (double)rs[safe]);
+        rate = valid ? (double)rain_rate[bt.rain_off[m] + safe] : 0.0;
     }
     const real dtr = (real)dt;
     ctl->dt_real = (double)dtr;
     ctl->dt = (double)dtr;
     ctl->rain_rate = (double)(real)rate;
     // parity_runner.py:105-115 -- external inflow interpolation at t + dt/2
-    if (n_ext_times > 0) {
+    const int n_ext = bt.ext_n[m];
+    if (n_ext > 0) {
+        const float* et = ext_times + bt.ext_off[m];
         const double te = t + 0.5 * (double)dtr;
-        int lo = 0, hi = n_ext_times;
-        while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)ext_times[mid] <= te) lo = mid + 1; else hi = mid; }
+        int lo = 0, hi = n_ext;
+        while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)et[mid] <= te) lo = mid + 1; else hi = mid; }
         const int upper = lo;
-        int lower = upper - 1; lower = lower < 0 ? 0 : (lower > n_ext_times - 1 ? n_ext_times - 1 : lower);
-        const int us = upper > n_ext_times - 1 ? n_ext_times - 1 : upper;
-        const double den = fmax((double)ext_times[us] - (double)ext_times[lower], 1.0e-12);
-        double frac = (te - (double)ext_times[lower]) / den;
+        int lower = upper - 1; lower = lower < 0 ? 0 : (lower > n_ext - 1 ? n_ext - 1 : lower);
+        const int us = upper > n_ext - 1 ? n_ext - 1 : upper;
+        const double den = fmax((double)et[us] - (double)et[lower], 1.0e-12);
+        double frac = (te - (double)et[lower]) / den;
         frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
         ctl->ext_lower = lower;
         ctl->ext_upper = us;
@@ -190,23 +215,37 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctl, KP<real> p
     ctl->interval_step += 1;
 }
 
+// Edge index space: Seg plus each member's range of blocks holding ghost edges.
+struct EdgeSeg {
+    Seg seg;
+    const int* gblk0;
+    const int* gnblk;
+};
+
 // ---------------------------------------------------------------------------
-// k_flux: one thread per kept edge.  Interior edges [0, n_int) first, ghost
-// edges [n_int, n_edges) after, so only the tail blocks reduce boundary volume.
-// Writes the edge record rec[e] = (m, fuL, fvL, fuR), recR[e] = fvR (all * length).
+// k_flux: EPT edges per thread (tile = blockDim * EPT, coalesced per k).  Each
+// member's edges are laid out interior first, then ghost edges, so only the
+// member's tail blocks reduce boundary volume.  Writes the edge record
+// rec[e] = (m, fuL, fvL, fuR), recR[e] = fvR (all * length).
 // ---------------------------------------------------------------------------
-template <typename real>
-__global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctl, KP<real> p, int n_int, int n_edges,
+template <typename real, int EPT>
+__global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es,
         const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
         const typename V4<real>::type* __restrict__ state,
         typename V4<real>::type* __restrict__ rec, real* __restrict__ recR, double* __restrict__ part) {
-    if (!ctl->active) return;
     typedef typename V4<real>::type V;
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
-    const int e = b * blockDim.x + threadIdx.x;
+    const int m = es.seg.blk_member[b];
+    Ctl* ctl = ctls + m;
+    if (!ctl->active) return;
+    const int end = es.seg.mem_end[m];
+    const real dt = (real)ctl->dt_real;
     double bin = 0.0, bout = 0.0;
-    if (e < n_edges) {
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int e = (b * EPT + k) * (int)blockDim.x + threadIdx.x;
+        if (e >= end) continue;
         const int l = el[e];
         const int r = er[e];
         const real nx = enx[e], ny = eny[e], len = elen[e];
@@ -229,21 +268,20 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctl, KP<real> p,
         if (r >= 0) {
             recR[e] = f.fvR;
         } else {
-            const double outward = (double)(f.m * (real)ctl->dt_real);
-            bin = fmax(-outward, 0.0);
-            bout = fmax(outward, 0.0);
+            const double outward = (double)(f.m * dt);
+            bin += fmax(-outward, 0.0);
+            bout += fmax(outward, 0.0);
         }
     }
-    const int first_ghost_block = n_int / (int)blockDim.x;
-    if (n_edges > n_int && b >= first_ghost_block) {
+    const int g0 = es.gblk0[m], gn = es.gnblk[m];
+    if (gn > 0 && b >= g0 && b < g0 + gn) {
         __shared__ double lds[16 * 2];
         double v[2] = {bin, bout};
         block_sum<2>(v, lds);
-        publish_partials<2>(part, b - first_ghost_block, v);
-        const int nparts = (int)gridDim.x - first_ghost_block;
-        if (!arrive_last(&ctl->counters[C_FLUX], nparts)) return;
+        publish_partials<2>(part, b, v);
+        if (!arrive_last(&ctl->counters[C_FLUX], gn)) return;
         double s[2];
-        sum_partials<2>(part, nparts, s, lds);
+        sum_partials<2>(part + 2 * (size_t)g0, gn, s, lds);
         if (threadIdx.x == 0) {
             ctl->counters[C_FLUX] = 0u;
             ctl->diag[L_BIN] = s[0];
@@ -274,26 +312,29 @@ __device__ __forceinline__ real two_sum(real a, real b, real& err) {
 // (:461-463).  COMP carries a per-cell TwoSum compensation of h, hu, hv.
 // ---------------------------------------------------------------------------
 template <typename real, bool COMP>
-__global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctl, KP<real> p, int n_cells,
+__global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real> p, Seg seg,
         const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
         const typename V4<real>::type* __restrict__ rec, const real* __restrict__ recR,
         typename V4<real>::type* __restrict__ state, typename V4<real>::type* __restrict__ comp,
         real* __restrict__ wet, const real* __restrict__ area_, const real* __restrict__ n2_,
         const real* __restrict__ rainm, const real* __restrict__ f0_, const real* __restrict__ fc_,
         const real* __restrict__ hk_, double* __restrict__ part) {
-    if (!ctl->active) return;
     typedef typename V4<real>::type V;
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    const int m = seg.blk_member[b];
+    Ctl* ctl = ctls + m;
+    if (!ctl->active) return;
     const int c = b * blockDim.x + threadIdx.x;
+    const int blk0 = seg.mem_blk0[m], nblk = seg.mem_nblk[m];
     const real dt = (real)ctl->dt_real;
     const real dt_wet = (real)ctl->dt_wet;
     const real rate = (real)ctl->rain_rate;
     const real zero = real(0), eps = p.eps;
     double rain_v = 0.0, inf_v = 0.0;
-    if (b == 0 && threadIdx.x == 0) {
+    if (b == blk0 && threadIdx.x == 0) {
         ctl->eval_next = (p.fixed_dt <= 0.0 && ctl->override_dt == 0 && (ctl->step % p.interval) == 0) ? 1 : 0;
     }
-    if (c < n_cells) {
+    if (c < seg.mem_end[m]) {
         V s = state[c];
         // deferred wet-time increment of the previous step (h here is its final depth)
         real wt = wet[c];
@@ -331,7 +372,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctl, KP<real> 
                 h = two_sum<real>(s.x, dt * dh / A + cc.x, ch);
                 hu = two_sum<real>(s.y, dt * dhu / A + cc.y, chu);
                 hv = two_sum<real>(s.z, dt * dhv / A + cc.z, chv);
-                if (!(h > zero)) { h = zero; ch = zero; }
+                if (!(h > zero)) { h = vmax(h, zero); ch = zero; }
             } else {
                 h = vmax(s.x + dt * dh / A, zero);
                 hu = s.y + dt * dhu / A;
@@ -382,9 +423,9 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctl, KP<real> 
     double v[2] = {rain_v, inf_v};
     block_sum<2>(v, lds);
     publish_partials<2>(part, b, v);
-    if (!arrive_last(&ctl->counters[C_UPDATE], gridDim.x)) return;
+    if (!arrive_last(&ctl->counters[C_UPDATE], nblk)) return;
     double sum[2];
-    sum_partials<2>(part, gridDim.x, sum, lds);
+    sum_partials<2>(part + 2 * (size_t)blk0, nblk, sum, lds);
     if (threadIdx.x == 0) {
         ctl->counters[C_UPDATE] = 0u;
         ctl->diag[L_RAIN] = sum[0];
@@ -394,20 +435,29 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctl, KP<real> 
     }
 }
 
-// Applies the pending wet-time increment of the last executed step.
+// Applies the pending wet-time increment of each member's last executed step.
 template <typename real>
-__global__ void k_finalize_wet(const Ctl* __restrict__ ctl, KP<real> p, int n_cells,
+__global__ void k_finalize_wet(const Ctl* __restrict__ ctls, KP<real> p, Seg seg,
                                const typename V4<real>::type* __restrict__ state, real* __restrict__ wet) {
+    const int b = blockIdx.x;
+    const int m = seg.blk_member[b];
+    const Ctl* ctl = ctls + m;
     if (!ctl->pending_wet) return;
+    const int c = b * blockDim.x + threadIdx.x;
+    if (c >= seg.mem_end[m]) return;
     const real dt = (real)ctl->dt_real;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells; c += gridDim.x * blockDim.x) {
-        if (state[c].x > p.eps) wet[c] = wet[c] + dt;
-    }
+    if (state[c].x > p.eps) wet[c] = wet[c] + dt;
 }
 
-__global__ void k_clear_pending(Ctl* ctl) { ctl->pending_wet = 0; }
+__global__ void k_clear_pending(Ctl* ctls, int members) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < members) ctls[m].pending_wet = 0;
+}
 
-__global__ void k_set_target(Ctl* ctl, double target, double final_t, int64_t max_steps) {
+__global__ void k_set_target(Ctl* ctls, int members, double target, double final_t, int64_t max_steps) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= members) return;
+    Ctl* ctl = ctls + m;
     ctl->target = target;
     ctl->final_t = final_t;
     ctl->max_steps = max_steps;
@@ -415,16 +465,20 @@ __global__ void k_set_target(Ctl* ctl, double target, double final_t, int64_t ma
     ctl->active = 0;
 }
 
-__global__ void k_set_override(Ctl* ctl, int on, double dt, double rain) {
+__global__ void k_set_override(Ctl* ctls, int members, int on, double dt, double rain) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= members) return;
+    Ctl* ctl = ctls + m;
     ctl->override_dt = on;
     ctl->override_dt_value = dt;
     ctl->override_rain = rain;
     if (on) ctl->eval_next = 0;
 }
 
-// Counts non-finite depths and node volumes (parity_runner.py:319-322 checks).
+// Counts non-finite depths and node volumes over the whole batch
+// (parity_runner.py:319-322 checks); the count lands in ctls[0].nonfinite.
 template <typename real>
-__global__ void k_finite(Ctl* __restrict__ ctl, int n_cells, const typename V4<real>::type* __restrict__ state,
+__global__ void k_finite(Ctl* __restrict__ ctls, int n_cells, const typename V4<real>::type* __restrict__ state,
                          int n_nodes, const real* __restrict__ vol, double* __restrict__ part) {
     __shared__ double lds[16];
     double bad = 0.0;
@@ -435,12 +489,12 @@ __global__ void k_finite(Ctl* __restrict__ ctl, int n_cells, const typename V4<r
     double r[1] = {bad};
     block_sum<1>(r, lds);
     publish_partials<1>(part, blockIdx.x, r);
-    if (!arrive_last(&ctl->counters[C_FINITE], gridDim.x)) return;
+    if (!arrive_last(&ctls[0].counters[C_FINITE], gridDim.x)) return;
     double s[1];
     sum_partials<1>(part, gridDim.x, s, lds);
     if (threadIdx.x == 0) {
-        ctl->counters[C_FINITE] = 0u;
-        ctl->nonfinite = (int)s[0];
+        ctls[0].counters[C_FINITE] = 0u;
+        ctls[0].nonfinite = (int)s[0];
     }
 }
 

@@ -1,11 +1,14 @@
 // floodhip: host engine and C ABI (see include/floodhip.h).
 //
-// One Engine<real> per context holds the device-resident scenario: SoA edge
-// geometry, packed cell state (h, hu, hv, bed), the cell->edge gather table,
-// drainage CSR tables, the step control block and the FP64 ledger.  A step is
-// a fixed sequence of kernel launches on the context stream; fh_advance_to
-// replays captured hipGraphs of steps_per_launch steps and reads the clock
-// back once per replay batch.
+// An Engine<real> holds a batch of M members (M = 1 for the plain solver):
+// independent scenarios, each with its own mesh, drainage network, forcing,
+// clock, dt and FP64 ledger.  Uploads stage each member on the host; pack()
+// concatenates the members into device arrays member by member, each member's
+// range of every index space starting on a tile boundary, and builds the
+// block -> member tables the kernels use.  A step is one fixed sequence of
+// kernel launches covering every member; fh_advance_to replays captured
+// hipGraphs of steps_per_launch steps and reads the control blocks back once
+// per replay batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <string>
 #include <vector>
 
@@ -52,16 +56,72 @@ struct DevArray {
         p = nullptr;
         n = 0;
     }
-    void upload(const T* host, size_t count, hipStream_t s) {
-        if (count != n) alloc(count);
-        if (count) FH_CHECK(hipMemcpyAsync(p, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    // uploads v (at least one element is always allocated so kernels get a valid pointer)
+    void set(const std::vector<T>& v, hipStream_t s) {
+        const size_t cnt = std::max<size_t>(v.size(), 1);
+        if (cnt != n) alloc(cnt);
+        if (!v.empty()) FH_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
     }
-    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
-    void download(T* host, size_t count, hipStream_t s) const {
-        if (count) FH_CHECK(hipMemcpyAsync(host, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    void put(const T* host, size_t offset, size_t count, hipStream_t s) {
+        if (count) FH_CHECK(hipMemcpyAsync(p + offset, host, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void get(T* host, size_t offset, size_t count, hipStream_t s) const {
+        if (count) FH_CHECK(hipMemcpyAsync(host, p + offset, count * sizeof(T), hipMemcpyDeviceToHost, s));
     }
     size_t bytes() const { return n * sizeof(T); }
     ~DevArray() { release(); }
+};
+
+// One index space of the batch: members concatenated on tile boundaries.
+struct Layout {
+    int tile = 256;
+    std::vector<int> base, count;              // per member (in pack order)
+    std::vector<int> blk_member, mem_blk0, mem_nblk, mem_end;
+    int total = 0, blocks = 0;
+    void build(const std::vector<int64_t>& counts, int tile_) {
+        tile = tile_;
+        const int M = (int)counts.size();
+        base.assign(M, 0); count.assign(M, 0);
+        mem_blk0.assign(M, 0); mem_nblk.assign(M, 0); mem_end.assign(M, 0);
+        blk_member.clear();
+        int64_t pos = 0;
+        for (int m = 0; m < M; ++m) {
+            const int64_t nb = (counts[m] + tile - 1) / tile;
+            base[m] = (int)pos;
+            count[m] = (int)counts[m];
+            mem_blk0[m] = (int)(pos / tile);
+            mem_nblk[m] = (int)nb;
+            mem_end[m] = (int)(pos + counts[m]);
+            for (int64_t b = 0; b < nb; ++b) blk_member.push_back(m);
+            pos += nb * tile;
+        }
+        if (pos > INT32_MAX / 2) throw ValueError{"batch too large for 32-bit indices"};
+        total = (int)pos;
+        blocks = (int)(pos / tile);
+    }
+};
+
+// Permute a per-member table from pack position order to member id order.
+template <typename T>
+std::vector<T> by_id(const std::vector<T>& by_pos, const std::vector<int>& order) {
+    std::vector<T> out(by_pos.size());
+    for (size_t i = 0; i < by_pos.size(); ++i) out[(size_t)order[i]] = by_pos[i];
+    return out;
+}
+
+// Device copy of a Layout; kernels index every per-member table by member id.
+struct DevSeg {
+    DevArray<int> blk_member, mem_blk0, mem_nblk, mem_end;
+    void set(const Layout& L, const std::vector<int>& order, hipStream_t s) {
+        std::vector<int> bm(L.blk_member.size());
+        for (size_t b = 0; b < bm.size(); ++b) bm[b] = order[(size_t)L.blk_member[b]];
+        if (bm.empty()) bm.push_back(0);
+        blk_member.set(bm, s);
+        mem_blk0.set(by_id(L.mem_blk0, order), s);
+        mem_nblk.set(by_id(L.mem_nblk, order), s);
+        mem_end.set(by_id(L.mem_end, order), s);
+    }
+    fh::Seg seg() const { return fh::Seg{blk_member.p, mem_blk0.p, mem_nblk.p, mem_end.p}; }
 };
 
 struct KernelTiming {
@@ -71,13 +131,71 @@ struct KernelTiming {
     int64_t launches = 0;
 };
 
+// Counting-sort CSR: rows[i] in [0, nrows) for items i (negative rows skipped);
+// items of a row stay in ascending item order.
+void build_csr(const std::vector<int>& rows, int nrows, std::vector<int>& ptr, std::vector<int>& idx) {
+    ptr.assign((size_t)nrows + 1, 0);
+    for (int r : rows)
+        if (r >= 0) ptr[(size_t)r + 1]++;
+    for (int r = 0; r < nrows; ++r) ptr[(size_t)r + 1] += ptr[r];
+    idx.assign((size_t)ptr[nrows], 0);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (size_t i = 0; i < rows.size(); ++i)
+        if (rows[i] >= 0) idx[(size_t)fill[rows[i]]++] = (int)i;
+}
+
+template <typename real>
+std::vector<real> conv(const float* src, int64_t n, float dflt = 0.0f) {
+    std::vector<real> v((size_t)std::max<int64_t>(n, 0));
+    for (int64_t i = 0; i < n; ++i) v[(size_t)i] = (real)(src ? src[i] : dflt);
+    return v;
+}
+std::vector<int> ints(const int32_t* src, int64_t n, int dflt = -1) {
+    std::vector<int> v((size_t)std::max<int64_t>(n, 0));
+    for (int64_t i = 0; i < n; ++i) v[(size_t)i] = src ? src[i] : dflt;
+    return v;
+}
+
+// Host staging of one member (local indices).
+template <typename real>
+struct Member {
+    bool has_surface = false;
+    int64_t C = 0, E = 0, E_int = 0;
+    std::vector<int> el, er;
+    std::vector<real> nx, ny, len;
+    std::vector<int> ce0, ce1, ce2;     // local edge positions, encoded
+    std::vector<real> area, n2, lc, bed;
+    std::vector<uint8_t> active;
+    std::vector<real> rainm, f0, fc, hk;
+    std::vector<float> rain_s, rain_e, rain_r;
+    // drainage
+    int64_t N = 0, I = 0, L = 0, S = 0, R = 0, Rcv = 0, T = 0;
+    std::vector<real> inv, As, dmax, oarea, ocoef, otw, ocap;
+    std::vector<int> nsc;
+    std::vector<int> icell, inode;
+    std::vector<real> capq, surq;
+    std::vector<int> lfrom, lto, geom;
+    std::vector<real> llen, man, lcap, linv, D, minor, rw, rh;
+    std::vector<int> scell, snode;
+    std::vector<real> sw;
+    int snorm = 0;
+    std::vector<int> rcell;
+    std::vector<real> rS, rcc, rrc, rstop;
+    std::vector<float> ext_t, ext_q;
+    std::vector<int> rcv;
+    // state (host copy; authoritative only while the batch is not packed)
+    std::vector<real> h, hu, hv, wet, V, Q, rstore;
+    fh::Ctl ctl{};
+};
+
 class EngineBase {
 public:
     virtual ~EngineBase() = default;
+    virtual void select(int m) = 0;
+    virtual int members() const = 0;
     virtual void upload_surface(int64_t C, int64_t E, const float* bed, const float* n, const float* area,
                                 const float* lc, const int32_t* el, const int32_t* er, const float* nx,
-                                const float* ny, const float* len, const int32_t* code,
-                                const uint8_t* active) = 0;
+                                const float* ny, const float* len, const int32_t* code, const uint8_t* active) = 0;
     virtual void upload_coupling(int64_t N, int64_t I, int64_t S, const int32_t* icell, const int32_t* inode,
                                  const int32_t* nsc, const float* inv, const float* As, const float* dmax,
                                  const float* capq, const float* surq, const int32_t* scell,
@@ -111,17 +229,21 @@ class Engine final : public EngineBase {
     typedef typename fh::V4<real>::type V;
 
 public:
-    explicit Engine(const fh_params& prm) : P(prm) {
+    Engine(const fh_params& prm, int members) : P(prm), M(members), mem((size_t)members) {
         FH_CHECK(hipSetDevice(P.device));
         FH_CHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        FH_CHECK(hipMalloc((void**)&ctl, sizeof(fh::Ctl)));
-        FH_CHECK(hipHostMalloc((void**)&ctl_h, sizeof(fh::Ctl), hipHostMallocDefault));
-        std::memset(ctl_h, 0, sizeof(fh::Ctl));
-        ctl_h->cached_cfl = 1.0;
-        ctl_h->eval_next = P.fixed_dt > 0.0 ? 0 : 1;
-        FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl), hipMemcpyHostToDevice, stream));
-        bs = std::max(64, std::min(1024, P.block_size > 0 ? P.block_size : 256));
+        FH_CHECK(hipMalloc((void**)&ctl, sizeof(fh::Ctl) * M));
+        FH_CHECK(hipHostMalloc((void**)&ctl_h, sizeof(fh::Ctl) * M, hipHostMallocDefault));
+        for (auto& m : mem) {
+            std::memset(&m.ctl, 0, sizeof(fh::Ctl));
+            m.ctl.cached_cfl = 1.0;
+            m.ctl.eval_next = P.fixed_dt > 0.0 ? 0 : 1;
+        }
+        for (int m = 0; m < M; ++m) ctl_h[m] = mem[m].ctl;
+        FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl) * M, hipMemcpyHostToDevice, stream));
+        bs = std::max(64, std::min(256, P.block_size > 0 ? P.block_size : 256));
         bs = (bs / 64) * 64;
+        ept = P.edges_per_thread == 2 ? 2 : (P.edges_per_thread == 4 ? 4 : 1);
         kp.g = (real)P.gravity;
         kp.eps = (real)P.min_depth;
         kp.half_g = (real)(0.5 * P.gravity);
@@ -149,307 +271,757 @@ public:
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    // ------------------------------------------------------------------ uploads
+    void select(int m) override {
+        if (m < 0 || m >= M) throw ValueError{"member index out of range"};
+        cur = m;
+    }
+    int members() const override { return M; }
+    Member<real>& cm() { return mem[(size_t)cur]; }
+
+    // ------------------------------------------------------------------ staging
+    void touch() {
+        if (packed && !dirty) pull_all();   // device state becomes the host copy before repacking
+        dirty = true;
+        drop_graph();
+    }
+
     void upload_surface(int64_t C_, int64_t E_, const float* bed, const float* n, const float* area,
                         const float* lc_, const int32_t* el, const int32_t* er, const float* nx,
                         const float* ny, const float* len, const int32_t* code,
                         const uint8_t* active) override {
-        if (C_ <= 0 || E_ < 0 || C_ > INT32_MAX / 2 || E_ > INT32_MAX / 2) throw ValueError{"surface sizes out of range"};
+        if (C_ <= 0 || E_ < 0 || C_ > INT32_MAX / 4 || E_ > INT32_MAX / 4) throw ValueError{"surface sizes out of range"};
         if (!bed || !n || !area || !lc_ || (E_ && (!el || !er || !nx || !ny || !len || !code)))
             throw ValueError{"surface arrays must not be NULL"};
-        drop_graph();
-        C = C_;
+        for (int64_t e = 0; e < E_; ++e)
+            if (el[e] < 0 || el[e] >= C_ || er[e] >= C_) throw ValueError{"edge cell index out of range"};
+        touch();
+        Member<real>& s = cm();
         auto act = [&](int64_t c) { return active ? active[c] != 0 : true; };
-        for (int64_t e = 0; e < E_; ++e) {
-            if (el[e] < 0 || el[e] >= C || er[e] >= C) throw ValueError{"edge cell index out of range"};
-        }
-        // classify and keep edges whose left cell is active
         std::vector<int64_t> interior, ghost;
-        std::vector<int32_t> ghost_kind;
-        interior.reserve(E_);
+        std::vector<int> ghost_kind;
         for (int64_t e = 0; e < E_; ++e) {
             const int32_t l = el[e], r = er[e];
             if (!act(l)) continue;
             const bool in = r >= 0;
             const bool ra = in && act(r);
-            if (in && ra) {
-                interior.push_back(e);
-                continue;
-            }
+            if (in && ra) { interior.push_back(e); continue; }
             const int32_t cd = (in && !ra) ? 0 : code[e];
-            int32_t kind;
+            int kind;
             if (cd == 0 || cd == 2) kind = fh::kGhostReflect;
             else if (!in && cd == 4) kind = fh::kGhostOutflow;
             else kind = fh::kGhostCopy;
             ghost.push_back(e);
             ghost_kind.push_back(kind);
         }
-        E_int = (int64_t)interior.size();
-        E = E_int + (int64_t)ghost.size();
-        std::vector<int32_t> pos_of(E_, -1);
-        std::vector<int32_t> hl(E), hr(E);
-        std::vector<real> hnx(E), hny(E), hlen(E);
-        for (int64_t i = 0; i < E_int; ++i) {
-            int64_t e = interior[i];
+        s.C = C_;
+        s.E_int = (int64_t)interior.size();
+        s.E = s.E_int + (int64_t)ghost.size();
+        std::vector<int32_t> pos_of((size_t)E_, -1);
+        s.el.assign(s.E, 0); s.er.assign(s.E, 0);
+        s.nx.assign(s.E, 0); s.ny.assign(s.E, 0); s.len.assign(s.E, 0);
+        for (int64_t i = 0; i < s.E; ++i) {
+            const bool gh = i >= s.E_int;
+            const int64_t e = gh ? ghost[i - s.E_int] : interior[i];
             pos_of[e] = (int32_t)i;
-            hl[i] = el[e]; hr[i] = er[e];
-            hnx[i] = (real)nx[e]; hny[i] = (real)ny[e]; hlen[i] = (real)len[e];
-        }
-        for (size_t j = 0; j < ghost.size(); ++j) {
-            int64_t e = ghost[j], i = E_int + (int64_t)j;
-            pos_of[e] = (int32_t)i;
-            hl[i] = el[e]; hr[i] = ghost_kind[j];
-            hnx[i] = (real)nx[e]; hny[i] = (real)ny[e]; hlen[i] = (real)len[e];
+            s.el[i] = el[e];
+            s.er[i] = gh ? ghost_kind[i - s.E_int] : er[e];
+            s.nx[i] = (real)nx[e]; s.ny[i] = (real)ny[e]; s.len[i] = (real)len[e];
         }
         // cell -> edge table: left-side entries (ascending original edge), then right-side
-        std::vector<int32_t> lent(C * 3, -1), rent(C * 3, -1);
-        std::vector<uint8_t> nl(C, 0), nr(C, 0);
+        std::vector<int> lent(C_ * 3, -1), rent(C_ * 3, -1);
+        std::vector<uint8_t> nl(C_, 0), nr(C_, 0);
         for (int64_t e = 0; e < E_; ++e) {
             const int32_t p = pos_of[e];
             if (p < 0) continue;
             const int32_t l = el[e];
             if (nl[l] + nr[l] >= 3) throw ValueError{"a cell has more than three edges"};
             lent[l * 3 + nl[l]++] = p << 1;
-            if (p < E_int) {
+            if (p < s.E_int) {
                 const int32_t r = er[e];
                 if (nl[r] + nr[r] >= 3) throw ValueError{"a cell has more than three edges"};
                 rent[r * 3 + nr[r]++] = (p << 1) | 1;
             }
         }
-        std::vector<int32_t> c0(C), c1(C), c2(C);
-        for (int64_t c = 0; c < C; ++c) {
-            int32_t t[3] = {-1, -1, -1};
+        s.ce0.assign(C_, -1); s.ce1.assign(C_, -1); s.ce2.assign(C_, -1);
+        for (int64_t c = 0; c < C_; ++c) {
+            int t[3] = {-1, -1, -1};
             int k = 0;
             for (int i = 0; i < nl[c]; ++i) t[k++] = lent[c * 3 + i];
             for (int i = 0; i < nr[c]; ++i) t[k++] = rent[c * 3 + i];
             if (!act(c)) t[0] = fh::kCellInactive;
-            c0[c] = t[0]; c1[c] = t[1]; c2[c] = t[2];
+            s.ce0[c] = t[0]; s.ce1[c] = t[1]; s.ce2[c] = t[2];
         }
-        std::vector<real> harea(C), hn2(C), hlc(C);
-        host_bed.assign(C, real(0));
-        for (int64_t c = 0; c < C; ++c) {
+        s.area.assign(C_, 0); s.n2.assign(C_, 0); s.lc.assign(C_, 0); s.bed.assign(C_, 0);
+        s.active.assign(C_, 1);
+        for (int64_t c = 0; c < C_; ++c) {
             const real nn = (real)n[c];
-            harea[c] = (real)area[c];
-            hn2[c] = nn * nn;
-            hlc[c] = (real)lc_[c];
-            host_bed[c] = (real)bed[c];
+            s.area[c] = (real)area[c];
+            s.n2[c] = nn * nn;
+            s.lc[c] = (real)lc_[c];
+            s.bed[c] = (real)bed[c];
+            s.active[c] = act(c) ? 1 : 0;
         }
-        host_active.assign(C, 1);
-        for (int64_t c = 0; c < C; ++c) host_active[c] = act(c) ? 1 : 0;
-        d_el.upload(hl, stream); d_er.upload(hr, stream);
-        d_nx.upload(hnx, stream); d_ny.upload(hny, stream); d_len.upload(hlen, stream);
-        d_ce0.upload(c0, stream); d_ce1.upload(c1, stream); d_ce2.upload(c2, stream);
-        d_area.upload(harea, stream); d_n2.upload(hn2, stream); d_lc.upload(hlc, stream);
-        d_rec.alloc(std::max<int64_t>(E, 1));
-        d_recR.alloc(std::max<int64_t>(E_int, 1));
-        d_state.alloc(C);
-        d_wet.alloc(C);
-        if (P.compensated) d_comp.alloc(C);
-        // zero state, bed in .w
-        std::vector<V> s(C);
-        for (int64_t c = 0; c < C; ++c) { s[c].x = 0; s[c].y = 0; s[c].z = 0; s[c].w = host_bed[c]; }
-        d_state.upload(s, stream);
-        std::vector<real> z(C, real(0));
-        d_wet.upload(z, stream);
-        if (P.compensated) {
-            std::vector<V> zc(C);
-            for (auto& v : zc) { v.x = 0; v.y = 0; v.z = 0; v.w = 0; }
-            d_comp.upload(zc, stream);
+        s.rainm.assign(C_, real(1));
+        for (int64_t c = 0; c < C_; ++c) if (!s.active[c]) s.rainm[c] = real(0);
+        s.f0.assign(C_, real(0)); s.fc.assign(C_, real(0)); s.hk.assign(C_, real(0));
+        s.h.assign(C_, real(0)); s.hu.assign(C_, real(0)); s.hv.assign(C_, real(0)); s.wet.assign(C_, real(0));
+        s.has_surface = true;
+    }
+
+    void upload_coupling(int64_t N_, int64_t I_, int64_t S_, const int32_t* icell_, const int32_t* inode_,
+                         const int32_t* nsc_, const float* inv_, const float* As_, const float* dmax_,
+                         const float* capq_, const float* surq_, const int32_t* scell_, const int32_t* snode_,
+                         const float* sw_, int32_t snorm_) override {
+        Member<real>& s = cm();
+        if (!s.has_surface) throw ValueError{"fh_upload_surface must be called first"};
+        if (N_ < 0 || I_ < 0 || S_ < 0) throw ValueError{"negative coupling sizes"};
+        if (N_ > 0 && (!inv_ || !As_ || !dmax_)) throw ValueError{"node arrays must not be NULL"};
+        if (I_ > 0 && (!icell_ || !inode_)) throw ValueError{"inlet arrays must not be NULL"};
+        if (S_ > 0 && (!scell_ || !snode_ || !sw_)) throw ValueError{"surcharge table arrays must not be NULL"};
+        for (int64_t i = 0; i < I_; ++i)
+            if (icell_[i] < 0 || icell_[i] >= s.C || inode_[i] < 0 || inode_[i] >= N_)
+                throw ValueError{"inlet cell/node index out of range"};
+        for (int64_t n = 0; n < N_; ++n)
+            if (nsc_ && nsc_[n] >= s.C) throw ValueError{"node_surface_cell out of range"};
+        for (int64_t k = 0; k < S_; ++k)
+            if (scell_[k] >= s.C || snode_[k] >= N_) throw ValueError{"surcharge table index out of range"};
+        touch();
+        s.N = N_; s.I = I_; s.S = S_;
+        s.inv = conv<real>(inv_, N_); s.As = conv<real>(As_, N_); s.dmax = conv<real>(dmax_, N_);
+        s.nsc = ints(nsc_, N_);
+        s.icell = ints(icell_, I_); s.inode = ints(inode_, I_);
+        s.capq = conv<real>(capq_, I_); s.surq = conv<real>(surq_, I_);
+        s.scell = ints(scell_, S_); s.snode = ints(snode_, S_); s.sw = conv<real>(sw_, S_);
+        s.snorm = snorm_ ? 1 : 0;
+        s.V.assign(N_, real(0));
+        if ((int64_t)s.oarea.size() != N_) {
+            s.oarea.assign(N_, real(0)); s.ocoef.assign(N_, real(0)); s.otw.assign(N_, real(0)); s.ocap.assign(N_, real(0));
         }
-        // default forcing: no rain multiplier effect (1), no infiltration
-        std::vector<real> ones(C, real(1));
-        d_rainm.upload(ones, stream);
-        d_f0.upload(z, stream); d_fc.upload(z, stream); d_k.upload(z, stream);
-        // launch geometry and reduction scratch
-        g_edge = (int)std::max<int64_t>(1, (E + bs - 1) / bs);
-        g_cell = (int)std::max<int64_t>(1, (C + bs - 1) / bs);
-        g_begin = (int)std::min<int64_t>(512, std::max<int64_t>(1, (C + 1023) / 1024));
-        d_part_begin.alloc(g_begin);
-        d_part_flux.alloc(2 * (size_t)g_edge);
-        d_part_upd.alloc(2 * (size_t)g_cell);
-        d_part_fin.alloc(1024);
-        if (N == 0) ensure_empty_drainage();
-        FH_CHECK(hipStreamSynchronize(stream));
-        surface_ready = true;
+        if (s.L > 0) {  // links refer to nodes: a new node set drops them
+            s.L = 0;
+            s.lfrom.clear(); s.lto.clear();
+        }
     }
 
-    void ensure_empty_drainage() {}
-
-    void upload_coupling(int64_t N_, int64_t I_, int64_t S_, const int32_t* icell, const int32_t* inode,
-                         const int32_t* nsc, const float* inv, const float* As, const float* dmax,
-                         const float* capq, const float* surq, const int32_t* scell, const int32_t* snode,
-                         const float* sw, int32_t snorm) override {
-        need_surface();
-        drop_graph();
-        drain.upload_coupling(*this, N_, I_, S_, icell, inode, nsc, inv, As, dmax, capq, surq, scell, snode, sw, snorm);
-        N = N_;
-        I = I_;
-        FH_CHECK(hipStreamSynchronize(stream));
+    void upload_pipe(int64_t L_, const int32_t* from, const int32_t* to, const float* len_, const float* n_,
+                     const float* cap_, const float* inv_, const float* D_, const float* minor_,
+                     const int32_t* geom_, const float* rw_, const float* rh_, const float* oa, const float* oc,
+                     const float* otw_, const float* ocap_) override {
+        Member<real>& s = cm();
+        if (!s.has_surface) throw ValueError{"fh_upload_surface must be called first"};
+        if (L_ < 0) throw ValueError{"negative link count"};
+        if (L_ > 0 && (!from || !to || !len_ || !n_ || !cap_ || !inv_ || !D_))
+            throw ValueError{"link arrays must not be NULL"};
+        for (int64_t l = 0; l < L_; ++l)
+            if (from[l] < 0 || from[l] >= s.N || to[l] < 0 || to[l] >= s.N)
+                throw ValueError{"link node index out of range (upload coupling first)"};
+        touch();
+        s.L = L_;
+        s.lfrom = ints(from, L_); s.lto = ints(to, L_); s.geom = ints(geom_, L_, 1);
+        s.llen = conv<real>(len_, L_); s.man = conv<real>(n_, L_); s.lcap = conv<real>(cap_, L_);
+        s.linv = conv<real>(inv_, L_); s.D = conv<real>(D_, L_); s.minor = conv<real>(minor_, L_);
+        s.rw = conv<real>(rw_, L_); s.rh = conv<real>(rh_, L_);
+        s.Q.assign(L_, real(0));
+        s.oarea = conv<real>(oa, s.N); s.ocoef = conv<real>(oc, s.N);
+        s.otw = conv<real>(otw_, s.N); s.ocap = conv<real>(ocap_, s.N);
     }
 
-    void upload_pipe(int64_t L_, const int32_t* from, const int32_t* to, const float* len, const float* n,
-                     const float* cap, const float* inv, const float* D, const float* minor, const int32_t* geom,
-                     const float* rw, const float* rh, const float* oa, const float* oc, const float* otw,
-                     const float* ocap) override {
-        need_surface();
-        drop_graph();
-        drain.upload_pipe(*this, L_, from, to, len, n, cap, inv, D, minor, geom, rw, rh, oa, oc, otw, ocap);
-        L = L_;
-        FH_CHECK(hipStreamSynchronize(stream));
-    }
-
-    void upload_recharge(int64_t R_, const int32_t* cell, const float* S, const float* cc, const float* rc,
+    void upload_recharge(int64_t R_, const int32_t* cell, const float* S_, const float* cc, const float* rc,
                          const float* stop) override {
-        need_surface();
-        drop_graph();
-        drain.upload_recharge(*this, R_, cell, S, cc, rc, stop);
-        R = R_;
-        FH_CHECK(hipStreamSynchronize(stream));
+        Member<real>& s = cm();
+        if (!s.has_surface) throw ValueError{"fh_upload_surface must be called first"};
+        if (R_ < 0) throw ValueError{"negative recharge row count"};
+        if (R_ > 0 && (!cell || !S_ || !cc || !rc || !stop)) throw ValueError{"recharge arrays must not be NULL"};
+        for (int64_t r = 0; r < R_; ++r)
+            if (cell[r] < 0 || cell[r] >= s.C) throw ValueError{"recharge source_cell out of range"};
+        touch();
+        s.R = R_;
+        s.rcell = ints(cell, R_);
+        s.rS = conv<real>(S_, R_); s.rcc = conv<real>(cc, R_); s.rrc = conv<real>(rc, R_);
+        s.rstop = conv<real>(stop, R_);
+        s.rstore.assign(R_, real(0));
     }
 
     void set_forcing(int64_t nr, const float* rs, const float* re, const float* rr, const float* m,
                      const float* f0, const float* fc, const float* k, int64_t nt, const float* et,
                      int64_t nrcv, const int32_t* rcv, const float* q) override {
-        need_surface();
-        drop_graph();
+        Member<real>& s = cm();
+        if (!s.has_surface) throw ValueError{"fh_upload_surface must be called first"};
         if (nr < 0 || nt < 0 || nrcv < 0) throw ValueError{"negative forcing sizes"};
         if (nr && (!rs || !re || !rr)) throw ValueError{"rain interval arrays must not be NULL"};
-        d_rain_s.upload(rs, (size_t)nr, stream);
-        d_rain_e.upload(re, (size_t)nr, stream);
-        d_rain_r.upload(rr, (size_t)nr, stream);
-        n_rain = nr;
-        std::vector<real> hm(C), hf0(C), hfc(C), hk(C);
-        for (int64_t c = 0; c < C; ++c) {
+        if (nrcv > 0 && (nt <= 0 || !et || !rcv || !q)) throw ValueError{"external inflow arrays incomplete"};
+        for (int64_t j = 0; j < nrcv; ++j)
+            if (rcv[j] < 0 || rcv[j] >= s.C) throw ValueError{"external receiver cell out of range"};
+        touch();
+        s.rain_s.assign(rs, rs + nr); s.rain_e.assign(re, re + nr); s.rain_r.assign(rr, rr + nr);
+        for (int64_t c = 0; c < s.C; ++c) {
             // multiplier folded with the active mask (rain_depth = rate * m * dt * active)
-            hm[c] = host_active[c] ? (real)(m ? m[c] : 1.0f) : real(0);
-            hf0[c] = (real)(f0 ? f0[c] : 0.0f);
-            hfc[c] = (real)(fc ? fc[c] : 0.0f);
-            hk[c] = (real)(k ? k[c] : 0.0f);
+            s.rainm[c] = s.active[c] ? (real)(m ? m[c] : 1.0f) : real(0);
+            s.f0[c] = (real)(f0 ? f0[c] : 0.0f);
+            s.fc[c] = (real)(fc ? fc[c] : 0.0f);
+            s.hk[c] = (real)(k ? k[c] : 0.0f);
         }
-        d_rainm.upload(hm, stream);
-        d_f0.upload(hf0, stream); d_fc.upload(hfc, stream); d_k.upload(hk, stream);
-        drain.set_external(*this, nt, et, nrcv, rcv, q);
-        n_ext_times = nrcv > 0 ? nt : 0;
-        FH_CHECK(hipStreamSynchronize(stream));
+        s.Rcv = nrcv;
+        s.T = nrcv > 0 ? nt : 0;
+        s.ext_t.assign(et ? et : nullptr, et ? et + s.T : nullptr);
+        s.ext_q.assign(q ? q : nullptr, q ? q + s.T * s.Rcv : nullptr);
+        s.rcv = ints(rcv, nrcv);
     }
 
     void set_state(const float* h, const float* hu, const float* hv, const float* wet, const float* vol,
                    const float* q, const float* rs) override {
-        need_surface();
-        std::vector<V> s(C);
-        d_state.download(s.data(), C, stream);
-        FH_CHECK(hipStreamSynchronize(stream));
-        for (int64_t c = 0; c < C; ++c) {
-            if (h) s[c].x = (real)h[c];
-            if (hu) s[c].y = (real)hu[c];
-            if (hv) s[c].z = (real)hv[c];
+        Member<real>& s = cm();
+        if (!s.has_surface) throw ValueError{"fh_upload_surface must be called first"};
+        if (packed && !dirty) {
+            // write the member's slices on the device
+            const int cb = cells.base[pos_of_member(cur)];
+            std::vector<V> st((size_t)s.C);
+            d_state.get(st.data(), cb, s.C, stream);
+            FH_CHECK(hipStreamSynchronize(stream));
+            for (int64_t c = 0; c < s.C; ++c) {
+                if (h) st[c].x = (real)h[c];
+                if (hu) st[c].y = (real)hu[c];
+                if (hv) st[c].z = (real)hv[c];
+            }
+            d_state.put(st.data(), cb, s.C, stream);
+            if (wet) { auto w = conv<real>(wet, s.C); d_wet.put(w.data(), cb, s.C, stream); }
+            if (P.compensated && (h || hu || hv)) {
+                std::vector<V> z((size_t)s.C);
+                for (auto& v : z) { v.x = 0; v.y = 0; v.z = 0; v.w = 0; }
+                d_comp.put(z.data(), cb, s.C, stream);
+            }
+            const int mp = pos_of_member(cur);
+            if (vol && s.N) { auto v = conv<real>(vol, s.N); d_V.put(v.data(), nodes.base[mp], s.N, stream); }
+            if (q && s.L) { auto v = conv<real>(q, s.L); d_Q.put(v.data(), links.base[mp], s.L, stream); }
+            if (rs && s.R) { auto v = conv<real>(rs, s.R); d_rstore.put(v.data(), rrow_base[mp], s.R, stream); }
+            FH_CHECK(hipStreamSynchronize(stream));
+            return;
         }
-        d_state.upload(s, stream);
-        if (wet) {
-            std::vector<real> w(wet, wet + C);
-            d_wet.upload(w, stream);
-        }
-        if (P.compensated && (h || hu || hv)) {
-            std::vector<V> zc(C);
-            for (auto& v : zc) { v.x = 0; v.y = 0; v.z = 0; v.w = 0; }
-            d_comp.upload(zc, stream);
-        }
-        drain.set_state(*this, vol, q, rs);
-        FH_CHECK(hipStreamSynchronize(stream));
+        if (h) s.h = conv<real>(h, s.C);
+        if (hu) s.hu = conv<real>(hu, s.C);
+        if (hv) s.hv = conv<real>(hv, s.C);
+        if (wet) s.wet = conv<real>(wet, s.C);
+        if (vol && s.N) s.V = conv<real>(vol, s.N);
+        if (q && s.L) s.Q = conv<real>(q, s.L);
+        if (rs && s.R) s.rstore = conv<real>(rs, s.R);
     }
-
-    void finalize_pending() {
-        fh::k_finalize_wet<real><<<g_fin(), 256, 0, stream>>>(ctl, kp, (int)C, d_state.p, d_wet.p);
-        fh::k_clear_pending<<<1, 1, 0, stream>>>(ctl);
-    }
-    int g_fin() const { return (int)std::min<int64_t>(1024, std::max<int64_t>(1, (C + 255) / 256)); }
 
     void get_state(float* h, float* hu, float* hv, float* wet, float* vol, float* q, float* rs) override {
-        need_surface();
-        std::vector<V> s(C);
-        d_state.download(s.data(), C, stream);
-        std::vector<real> w(C);
-        d_wet.download(w.data(), C, stream);
+        ensure_packed();
+        Member<real>& s = cm();
+        const int mp = pos_of_member(cur);
+        const int cb = cells.base[mp];
+        std::vector<V> st((size_t)s.C);
+        std::vector<real> w((size_t)s.C), v((size_t)s.N), qq((size_t)s.L), r((size_t)s.R);
+        d_state.get(st.data(), cb, s.C, stream);
+        d_wet.get(w.data(), cb, s.C, stream);
+        if (s.N) d_V.get(v.data(), nodes.base[mp], s.N, stream);
+        if (s.L) d_Q.get(qq.data(), links.base[mp], s.L, stream);
+        if (s.R) d_rstore.get(r.data(), rrow_base[mp], s.R, stream);
         FH_CHECK(hipStreamSynchronize(stream));
-        for (int64_t c = 0; c < C; ++c) {
-            if (h) h[c] = (float)s[c].x;
-            if (hu) hu[c] = (float)s[c].y;
-            if (hv) hv[c] = (float)s[c].z;
+        for (int64_t c = 0; c < s.C; ++c) {
+            if (h) h[c] = (float)st[c].x;
+            if (hu) hu[c] = (float)st[c].y;
+            if (hv) hv[c] = (float)st[c].z;
             if (wet) wet[c] = (float)w[c];
         }
-        drain.get_state(*this, vol, q, rs);
+        if (vol) for (int64_t n = 0; n < s.N; ++n) vol[n] = (float)v[n];
+        if (q) for (int64_t l = 0; l < s.L; ++l) q[l] = (float)qq[l];
+        if (rs) for (int64_t k = 0; k < s.R; ++k) rs[k] = (float)r[k];
     }
 
     void get_state_f64(double* h, double* hu, double* hv) override {
-        need_surface();
-        std::vector<V> s(C);
-        d_state.download(s.data(), C, stream);
+        ensure_packed();
+        Member<real>& s = cm();
+        std::vector<V> st((size_t)s.C);
+        d_state.get(st.data(), cells.base[pos_of_member(cur)], s.C, stream);
         FH_CHECK(hipStreamSynchronize(stream));
-        for (int64_t c = 0; c < C; ++c) {
-            if (h) h[c] = (double)s[c].x;
-            if (hu) hu[c] = (double)s[c].y;
-            if (hv) hv[c] = (double)s[c].z;
+        for (int64_t c = 0; c < s.C; ++c) {
+            if (h) h[c] = (double)st[c].x;
+            if (hu) hu[c] = (double)st[c].y;
+            if (hv) hv[c] = (double)st[c].z;
         }
     }
 
+    // ------------------------------------------------------------------ control blocks
     void pull_ctl() {
-        FH_CHECK(hipMemcpyAsync(ctl_h, ctl, sizeof(fh::Ctl), hipMemcpyDeviceToHost, stream));
+        FH_CHECK(hipMemcpyAsync(ctl_h, ctl, sizeof(fh::Ctl) * M, hipMemcpyDeviceToHost, stream));
         FH_CHECK(hipStreamSynchronize(stream));
     }
     void push_ctl() {
-        FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl), hipMemcpyHostToDevice, stream));
+        FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl) * M, hipMemcpyHostToDevice, stream));
         FH_CHECK(hipStreamSynchronize(stream));
     }
-
+    // ctl_h is indexed by member id (not pack position)
     void set_clock(double t, double cached, int64_t step) override {
         pull_ctl();
-        ctl_h->t = t;
-        ctl_h->cached_cfl = (double)(real)cached;
-        ctl_h->step = step;
-        ctl_h->eval_next = (P.fixed_dt <= 0.0 && (step % kp.interval) == 0) ? 1 : 0;
+        fh::Ctl& c = ctl_h[cur];
+        c.t = t;
+        c.cached_cfl = (double)(real)cached;
+        c.step = step;
+        c.eval_next = (P.fixed_dt <= 0.0 && (step % kp.interval) == 0) ? 1 : 0;
         push_ctl();
     }
-
     void get_clock(fh_clock* c) override {
         pull_ctl();
-        c->time_s = ctl_h->t;
-        c->cached_dt_s = ctl_h->cached_cfl;
-        c->last_dt_s = ctl_h->dt;
-        c->global_step = ctl_h->step;
-        c->interval_steps = ctl_h->interval_step;
-        c->cfl_evaluations = ctl_h->cfl_evals;
+        const fh::Ctl& x = ctl_h[cur];
+        c->time_s = x.t;
+        c->cached_dt_s = x.cached_cfl;
+        c->last_dt_s = x.dt;
+        c->global_step = x.step;
+        c->interval_steps = x.interval_step;
+        c->cfl_evaluations = x.cfl_evals;
     }
-
     void set_ledger(const double* l) override {
         pull_ctl();
-        for (int k = 0; k < fh::kLedgerTerms; ++k) ctl_h->ledger[k] = l ? l[k] : 0.0;
+        for (int k = 0; k < fh::kLedgerTerms; ++k) ctl_h[cur].ledger[k] = l ? l[k] : 0.0;
         push_ctl();
     }
     void get_ledger(double* l) override {
         pull_ctl();
-        for (int k = 0; k < fh::kLedgerTerms; ++k) l[k] = ctl_h->ledger[k];
+        for (int k = 0; k < fh::kLedgerTerms; ++k) l[k] = ctl_h[cur].ledger[k];
+    }
+
+    // ------------------------------------------------------------------ packing
+    int pos_of_member(int m) const { return member_pos[(size_t)m]; }
+
+    void pull_all() {
+        // copy device state back into every member's host staging
+        for (int m = 0; m < M; ++m) {
+            Member<real>& s = mem[(size_t)m];
+            if (!s.has_surface) continue;
+            const int mp = pos_of_member(m);
+            std::vector<V> st((size_t)s.C);
+            d_state.get(st.data(), cells.base[mp], s.C, stream);
+            d_wet.get(s.wet.data(), cells.base[mp], s.C, stream);
+            if (s.N && (int64_t)s.V.size() == s.N) d_V.get(s.V.data(), nodes.base[mp], s.N, stream);
+            if (s.L && (int64_t)s.Q.size() == s.L) d_Q.get(s.Q.data(), links.base[mp], s.L, stream);
+            if (s.R && (int64_t)s.rstore.size() == s.R) d_rstore.get(s.rstore.data(), rrow_base[mp], s.R, stream);
+            FH_CHECK(hipStreamSynchronize(stream));
+            for (int64_t c = 0; c < s.C; ++c) { s.h[c] = st[c].x; s.hu[c] = st[c].y; s.hv[c] = st[c].z; }
+        }
+    }
+
+    void ensure_packed() {
+        for (int m = 0; m < M; ++m)
+            if (!mem[(size_t)m].has_surface) throw ValueError{"fh_upload_surface must be called for every member"};
+        if (!packed || dirty) pack();
+    }
+
+    void pack() {
+        drop_graph();
+        // member order in the concatenation
+        order.resize(M);
+        std::iota(order.begin(), order.end(), 0);
+        if (P.pack_order == 1) {
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return mem[a].C > mem[b].C; });
+        } else if (P.pack_order == 2) {   // alternate largest / smallest
+            std::vector<int> bysize(order);
+            std::stable_sort(bysize.begin(), bysize.end(), [&](int a, int b) { return mem[a].C > mem[b].C; });
+            int lo = 0, hi = M - 1, k = 0;
+            while (lo <= hi) { order[k++] = bysize[lo++]; if (lo <= hi) order[k++] = bysize[hi--]; }
+        }
+        member_pos.assign(M, 0);
+        for (int i = 0; i < M; ++i) member_pos[order[i]] = i;
+        auto counts = [&](auto fn) {
+            std::vector<int64_t> v(M);
+            for (int i = 0; i < M; ++i) v[i] = fn(mem[order[i]]);
+            return v;
+        };
+        cells.build(counts([](const Member<real>& s) { return s.C; }), bs);
+        edges.build(counts([](const Member<real>& s) { return s.E; }), bs * ept);
+        nodes.build(counts([](const Member<real>& s) { return s.N; }), 256);
+        links.build(counts([](const Member<real>& s) { return s.L; }), 256);
+        // per-member derived lists (inlet cells, return cells, receiver cells)
+        std::vector<std::vector<int>> icells(M), rcells(M), xcells(M);
+        for (int i = 0; i < M; ++i) {
+            const Member<real>& s = mem[order[i]];
+            std::vector<char> mark((size_t)s.C, 0);
+            for (int64_t k = 0; k < s.I; ++k) mark[s.icell[k]] = 1;
+            for (int64_t c = 0; c < s.C; ++c) if (mark[c]) icells[i].push_back((int)c);
+            std::fill(mark.begin(), mark.end(), 0);
+            for (int64_t k = 0; k < s.S; ++k) if (s.snode[k] >= 0 && s.scell[k] >= 0) mark[s.scell[k]] = 1;
+            std::vector<int> expl((size_t)s.N, 0);
+            for (int64_t k = 0; k < s.S; ++k) if (s.snode[k] >= 0 && s.scell[k] >= 0) expl[s.snode[k]] = 1;
+            for (int64_t n = 0; n < s.N; ++n) if (!expl[n] && s.nsc[n] >= 0) mark[s.nsc[n]] = 1;
+            for (int64_t r = 0; r < s.R; ++r) mark[s.rcell[r]] = 1;
+            for (int64_t c = 0; c < s.C; ++c) if (mark[c]) rcells[i].push_back((int)c);
+            std::fill(mark.begin(), mark.end(), 0);
+            for (int64_t j = 0; j < s.Rcv; ++j) mark[s.rcv[j]] = 1;
+            for (int64_t c = 0; c < s.C; ++c) if (mark[c]) xcells[i].push_back((int)c);
+        }
+        icl.build(counts_of(icells), 256);
+        rcl.build(counts_of(rcells), 256);
+        xcl.build(counts_of(xcells), 256);
+        // k_begin blocks: proportional to the member's cells
+        {
+            std::vector<int64_t> nb(M);
+            for (int i = 0; i < M; ++i) nb[i] = std::max<int64_t>(1, std::min<int64_t>(256, (mem[order[i]].C + 2047) / 2048));
+            begin_l.base.assign(M, 0); begin_l.count.assign(M, 0);
+            begin_l.blk_member.clear(); begin_l.mem_blk0.assign(M, 0); begin_l.mem_nblk.assign(M, 0);
+            begin_l.mem_end.assign(M, 0);
+            int b = 0;
+            for (int i = 0; i < M; ++i) {
+                begin_l.mem_blk0[i] = b;
+                begin_l.mem_nblk[i] = (int)nb[i];
+                for (int k = 0; k < nb[i]; ++k) begin_l.blk_member.push_back(i);
+                b += (int)nb[i];
+            }
+            begin_l.blocks = b;
+            begin_l.tile = 256;
+        }
+        const hipStream_t s = stream;
+        // ---------------- surface arrays
+        std::vector<int> el(edges.total, 0), er(edges.total, fh::kGhostCopy);
+        std::vector<real> nx(edges.total, 0), ny(edges.total, 0), len(edges.total, 0);
+        std::vector<int> gblk0(M, 0), gnblk(M, 0);
+        std::vector<int> ce0(cells.total, fh::kCellInactive), ce1(cells.total, -1), ce2(cells.total, -1);
+        std::vector<V> st(cells.total);
+        std::vector<real> wet(cells.total, 0), area(cells.total, 1), n2(cells.total, 0), lc(cells.total, 0),
+            rainm(cells.total, 0), f0(cells.total, 0), fc(cells.total, 0), hk(cells.total, 0);
+        for (auto& v : st) { v.x = 0; v.y = 0; v.z = 0; v.w = 0; }
+        std::vector<int> cell0(M), cell1(M), rain_off(M), rain_n(M), ext_off(M), ext_n(M);
+        std::vector<float> rs_all, re_all, rr_all, et_all;
+        for (int i = 0; i < M; ++i) {
+            const Member<real>& m = mem[order[i]];
+            const int cb = cells.base[i], eb = edges.base[i];
+            for (int64_t e = 0; e < m.E; ++e) {
+                el[eb + e] = m.el[e] + cb;
+                er[eb + e] = m.er[e] >= 0 ? m.er[e] + cb : m.er[e];
+                nx[eb + e] = m.nx[e]; ny[eb + e] = m.ny[e]; len[eb + e] = m.len[e];
+            }
+            const int tile = edges.tile;
+            if (m.E > m.E_int) {
+                gblk0[i] = (eb + (int)m.E_int) / tile;
+                gnblk[i] = (eb + (int)m.E - 1) / tile + 1 - gblk0[i];
+            }
+            auto enc = [&](int x) { return x < 0 ? x : ((((x >> 1) + eb) << 1) | (x & 1)); };
+            for (int64_t c = 0; c < m.C; ++c) {
+                ce0[cb + c] = enc(m.ce0[c]); ce1[cb + c] = enc(m.ce1[c]); ce2[cb + c] = enc(m.ce2[c]);
+                st[cb + c].x = m.h[c]; st[cb + c].y = m.hu[c]; st[cb + c].z = m.hv[c]; st[cb + c].w = m.bed[c];
+                wet[cb + c] = m.wet[c]; area[cb + c] = m.area[c]; n2[cb + c] = m.n2[c]; lc[cb + c] = m.lc[c];
+                rainm[cb + c] = m.rainm[c]; f0[cb + c] = m.f0[c]; fc[cb + c] = m.fc[c]; hk[cb + c] = m.hk[c];
+            }
+            cell0[i] = cb; cell1[i] = cb + (int)m.C;
+            rain_off[i] = (int)rs_all.size(); rain_n[i] = (int)m.rain_s.size();
+            rs_all.insert(rs_all.end(), m.rain_s.begin(), m.rain_s.end());
+            re_all.insert(re_all.end(), m.rain_e.begin(), m.rain_e.end());
+            rr_all.insert(rr_all.end(), m.rain_r.begin(), m.rain_r.end());
+            ext_off[i] = (int)et_all.size(); ext_n[i] = m.Rcv > 0 ? (int)m.T : 0;
+            if (m.Rcv > 0) et_all.insert(et_all.end(), m.ext_t.begin(), m.ext_t.end());
+        }
+        d_el.set(el, s); d_er.set(er, s); d_nx.set(nx, s); d_ny.set(ny, s); d_len.set(len, s);
+        d_rec.alloc(std::max(edges.total, 1)); d_recR.alloc(std::max(edges.total, 1));
+        d_ce0.set(ce0, s); d_ce1.set(ce1, s); d_ce2.set(ce2, s);
+        d_state.set(st, s); d_wet.set(wet, s); d_area.set(area, s); d_n2.set(n2, s); d_lc.set(lc, s);
+        d_rainm.set(rainm, s); d_f0.set(f0, s); d_fc.set(fc, s); d_k.set(hk, s);
+        if (P.compensated) {
+            std::vector<V> z(cells.total);
+            for (auto& v : z) { v.x = 0; v.y = 0; v.z = 0; v.w = 0; }
+            d_comp.set(z, s);
+        }
+        d_rain_s.set(rs_all, s); d_rain_e.set(re_all, s); d_rain_r.set(rr_all, s); d_ext_t.set(et_all, s);
+        seg_cells.set(cells, order, s); seg_edges.set(edges, order, s); seg_begin.set(begin_l, order, s);
+        d_gblk0.set(by_id(gblk0, order), s); d_gnblk.set(by_id(gnblk, order), s);
+        d_cell0.set(by_id(cell0, order), s); d_cell1.set(by_id(cell1, order), s);
+        d_rain_off.set(by_id(rain_off, order), s); d_rain_n.set(by_id(rain_n, order), s);
+        d_ext_off.set(by_id(ext_off, order), s); d_ext_n.set(by_id(ext_n, order), s);
+        d_part_begin.alloc(std::max(begin_l.blocks, 1));
+        d_part_flux.alloc(2 * (size_t)std::max(edges.blocks, 1));
+        d_part_upd.alloc(2 * (size_t)std::max(cells.blocks, 1));
+        d_part_fin.alloc(1024);
+        pack_drainage(icells, rcells, xcells);
+        // control blocks: per member, unchanged
+        FH_CHECK(hipStreamSynchronize(s));
+        packed = true;
+        dirty = false;
+        any_drainage = nodes.total > 0;
+    }
+
+    static std::vector<int64_t> counts_of(const std::vector<std::vector<int>>& v) {
+        std::vector<int64_t> c(v.size());
+        for (size_t i = 0; i < v.size(); ++i) c[i] = (int64_t)v[i].size();
+        return c;
+    }
+
+    void pack_drainage(const std::vector<std::vector<int>>& icells, const std::vector<std::vector<int>>& rcells,
+                       const std::vector<std::vector<int>>& xcells) {
+        const hipStream_t s = stream;
+        const int NT = nodes.total, LT = links.total;
+        // global item bases for inlets, entries, recharge rows, receivers (no padding needed)
+        std::vector<int> ibase(M), sbase(M), rbase(M), xbase(M), qbase(M);
+        int It = 0, St = 0, Rt = 0, Xt = 0, Qt = 0;
+        for (int i = 0; i < M; ++i) {
+            const Member<real>& m = mem[order[i]];
+            ibase[i] = It; It += (int)m.I;
+            sbase[i] = St; St += (int)m.S;
+            rbase[i] = Rt; Rt += (int)m.R;
+            xbase[i] = Xt; Xt += (int)m.Rcv;
+            qbase[i] = Qt; Qt += (int)(m.T * m.Rcv);
+        }
+        rrow_base = rbase;
+        std::vector<real> inv(NT, 0), As(NT, 1), dmax(NT, 0), oa(NT, 0), oc(NT, 0), otw(NT, 0), ocap(NT, 0), V(NT, 0);
+        std::vector<int> nsc(NT, -1), expl(NT, 0);
+        std::vector<int> ni_ptr(NT + 1, 0), nl_ptr(NT + 1, 0), ne_ptr(NT + 1, 0);
+        std::vector<int> ni_idx, nl_ent, ne_idx;
+        std::vector<int> icell(std::max(It, 0)), inode(std::max(It, 0));
+        std::vector<real> capq(It), surq(It);
+        std::vector<int> from(LT, 0), to(LT, 0), geom(LT, 1);
+        std::vector<real> llen(LT, 1), man(LT, 0), lcap(LT, 0), linv(LT, 0), D(LT, 1), minor(LT, 0), rw(LT, 0),
+            rh(LT, 0), Q(LT, 0);
+        std::vector<int> snode(St, -1);
+        std::vector<real> sw(St, 0);
+        std::vector<int> snorm(M, 0);
+        std::vector<real> rS(Rt), rcc(Rt), rrc(Rt), rstop(Rt), rstore(Rt);
+        std::vector<float> q_all;
+        std::vector<int> rcv_qbase(Xt), rcv_stride(Xt);
+        std::vector<int> ic_cells(icl.total, 0), ic_ptr(icl.total + 1, 0), ic_idx;
+        std::vector<int> rc_cells(rcl.total, 0), rc_eptr(rcl.total + 1, 0), rc_eidx, rc_fptr(rcl.total + 1, 0), rc_fidx,
+            rc_rptr(rcl.total + 1, 0), rc_ridx;
+        std::vector<int> xc_cells(xcl.total, 0), xc_ptr(xcl.total + 1, 0), xc_idx;
+        for (int i = 0; i < M; ++i) {
+            const Member<real>& m = mem[order[i]];
+            const int cb = cells.base[i], nb = nodes.base[i], lb = links.base[i];
+            snorm[i] = m.snorm;
+            for (int64_t n = 0; n < m.N; ++n) {
+                inv[nb + n] = m.inv[n]; As[nb + n] = m.As[n]; dmax[nb + n] = m.dmax[n];
+                oa[nb + n] = m.oarea[n]; oc[nb + n] = m.ocoef[n]; otw[nb + n] = m.otw[n]; ocap[nb + n] = m.ocap[n];
+                V[nb + n] = m.V[n];
+                nsc[nb + n] = m.nsc[n] >= 0 ? m.nsc[n] + cb : -1;
+            }
+            for (int64_t k = 0; k < m.I; ++k) {
+                icell[ibase[i] + k] = m.icell[k] + cb;
+                inode[ibase[i] + k] = m.inode[k] + nb;
+                capq[ibase[i] + k] = m.capq[k]; surq[ibase[i] + k] = m.surq[k];
+            }
+            for (int64_t l = 0; l < m.L; ++l) {
+                from[lb + l] = m.lfrom[l] + nb; to[lb + l] = m.lto[l] + nb; geom[lb + l] = m.geom[l];
+                llen[lb + l] = m.llen[l]; man[lb + l] = m.man[l]; lcap[lb + l] = m.lcap[l]; linv[lb + l] = m.linv[l];
+                D[lb + l] = m.D[l]; minor[lb + l] = m.minor[l]; rw[lb + l] = m.rw[l]; rh[lb + l] = m.rh[l];
+                Q[lb + l] = m.Q[l];
+            }
+            for (int64_t k = 0; k < m.S; ++k) {
+                const bool valid = m.snode[k] >= 0 && m.scell[k] >= 0;
+                snode[sbase[i] + k] = valid ? m.snode[k] + nb : -1;
+                sw[sbase[i] + k] = m.sw[k];
+            }
+            for (int64_t r = 0; r < m.R; ++r) {
+                rS[rbase[i] + r] = m.rS[r]; rcc[rbase[i] + r] = m.rcc[r]; rrc[rbase[i] + r] = m.rrc[r];
+                rstop[rbase[i] + r] = m.rstop[r]; rstore[rbase[i] + r] = m.rstore[r];
+            }
+            for (int64_t j = 0; j < m.Rcv; ++j) {
+                rcv_qbase[xbase[i] + j] = qbase[i] + (int)j;
+                rcv_stride[xbase[i] + j] = (int)m.Rcv;
+            }
+            q_all.insert(q_all.end(), m.ext_q.begin(), m.ext_q.end());
+        }
+        // node CSRs (global): node -> inlets, node -> links (ascending link, orientation bit), node -> entries
+        {
+            std::vector<int> rows(It);
+            for (int k = 0; k < It; ++k) rows[k] = inode[k];
+            build_csr(rows, NT, ni_ptr, ni_idx);
+            std::vector<int> cnt(NT + 1, 0);
+            for (int i = 0; i < M; ++i) {
+                const Member<real>& m = mem[order[i]];
+                for (int64_t l = 0; l < m.L; ++l) { cnt[from[links.base[i] + l] + 1]++; cnt[to[links.base[i] + l] + 1]++; }
+            }
+            for (int n = 0; n < NT; ++n) cnt[n + 1] += cnt[n];
+            nl_ent.assign(cnt[NT], 0);
+            std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+            for (int i = 0; i < M; ++i) {
+                const Member<real>& m = mem[order[i]];
+                for (int64_t l = 0; l < m.L; ++l) {
+                    const int g = links.base[i] + (int)l;
+                    nl_ent[fill[from[g]]++] = g << 1;
+                    nl_ent[fill[to[g]]++] = (g << 1) | 1;
+                }
+            }
+            nl_ptr = cnt;
+            build_csr(snode, NT, ne_ptr, ne_idx);
+            for (int n = 0; n < NT; ++n) expl[n] = ne_ptr[n + 1] > ne_ptr[n] ? 1 : 0;
+        }
+        // inlet cells -> inlets
+        for (int i = 0; i < M; ++i) {
+            const Member<real>& m = mem[order[i]];
+            std::vector<int> slot((size_t)m.C, -1);
+            for (size_t j = 0; j < icells[i].size(); ++j) {
+                slot[icells[i][j]] = icl.base[i] + (int)j;
+                ic_cells[icl.base[i] + j] = icells[i][j] + cells.base[i];
+            }
+            std::vector<int> rows(It, -1);
+            (void)rows;
+            for (int64_t k = 0; k < m.I; ++k) ic_ptr[slot[m.icell[k]] + 1]++;
+        }
+        for (int j = 0; j < icl.total; ++j) ic_ptr[j + 1] += ic_ptr[j];
+        ic_idx.assign(ic_ptr[icl.total], 0);
+        {
+            std::vector<int> fill(ic_ptr.begin(), ic_ptr.end() - 1);
+            for (int i = 0; i < M; ++i) {
+                const Member<real>& m = mem[order[i]];
+                std::vector<int> slot((size_t)m.C, -1);
+                for (size_t j = 0; j < icells[i].size(); ++j) slot[icells[i][j]] = icl.base[i] + (int)j;
+                for (int64_t k = 0; k < m.I; ++k) ic_idx[fill[slot[m.icell[k]]]++] = ibase[i] + (int)k;
+            }
+        }
+        // return cells -> entries, fallback nodes, recharge rows; receiver cells -> receivers
+        {
+            std::vector<int> erow(St, -1), frow(NT, -1), rrow(Rt, -1), xrow(Xt, -1);
+            for (int i = 0; i < M; ++i) {
+                const Member<real>& m = mem[order[i]];
+                std::vector<int> slot((size_t)m.C, -1), xslot((size_t)m.C, -1);
+                for (size_t j = 0; j < rcells[i].size(); ++j) {
+                    slot[rcells[i][j]] = rcl.base[i] + (int)j;
+                    rc_cells[rcl.base[i] + j] = rcells[i][j] + cells.base[i];
+                }
+                for (size_t j = 0; j < xcells[i].size(); ++j) {
+                    xslot[xcells[i][j]] = xcl.base[i] + (int)j;
+                    xc_cells[xcl.base[i] + j] = xcells[i][j] + cells.base[i];
+                }
+                for (int64_t k = 0; k < m.S; ++k)
+                    if (m.snode[k] >= 0 && m.scell[k] >= 0) erow[sbase[i] + k] = slot[m.scell[k]];
+                for (int64_t n = 0; n < m.N; ++n) {
+                    const int g = nodes.base[i] + (int)n;
+                    if (!expl[g] && m.nsc[n] >= 0) frow[g] = slot[m.nsc[n]];
+                }
+                for (int64_t r = 0; r < m.R; ++r) rrow[rbase[i] + r] = slot[m.rcell[r]];
+                for (int64_t j = 0; j < m.Rcv; ++j) xrow[xbase[i] + j] = xslot[m.rcv[j]];
+            }
+            build_csr(erow, rcl.total, rc_eptr, rc_eidx);
+            build_csr(frow, rcl.total, rc_fptr, rc_fidx);
+            build_csr(rrow, rcl.total, rc_rptr, rc_ridx);
+            build_csr(xrow, xcl.total, xc_ptr, xc_idx);
+        }
+        d_inv.set(inv, s); d_As.set(As, s); d_dmax.set(dmax, s); d_oa.set(oa, s); d_oc.set(oc, s); d_otw.set(otw, s);
+        d_ocap.set(ocap, s); d_V.set(V, s); d_nsc.set(nsc, s); d_expl.set(expl, s);
+        std::vector<real> zN(NT, 0);
+        d_sfac.set(zN, s); d_over.set(zN, s); d_fb.set(zN, s); d_wsum.set(zN, s);
+        d_ni_ptr.set(ni_ptr, s); d_ni_idx.set(ni_idx, s); d_nl_ptr.set(nl_ptr, s); d_nl_ent.set(nl_ent, s);
+        d_ne_ptr.set(ne_ptr, s); d_ne_idx.set(ne_idx, s);
+        d_icell.set(icell, s); d_inode.set(inode, s); d_capq.set(capq, s); d_surq.set(surq, s);
+        std::vector<real> zI(It, 0);
+        d_sur.set(zI, s); d_cap.set(zI, s);
+        d_ic_cells.set(ic_cells, s); d_ic_ptr.set(ic_ptr, s); d_ic_idx.set(ic_idx, s);
+        d_from.set(from, s); d_to.set(to, s); d_geom.set(geom, s); d_llen.set(llen, s); d_man.set(man, s);
+        d_lcap.set(lcap, s); d_linv.set(linv, s); d_D.set(D, s); d_minor.set(minor, s); d_rw.set(rw, s);
+        d_rh.set(rh, s); d_Q.set(Q, s);
+        std::vector<real> zL(LT, 0);
+        d_req.set(zL, s);
+        d_snode.set(snode, s); d_sw.set(sw, s); d_snorm.set(by_id(snorm, order), s);
+        d_rc_cells.set(rc_cells, s); d_rc_eptr.set(rc_eptr, s); d_rc_eidx.set(rc_eidx, s);
+        d_rc_fptr.set(rc_fptr, s); d_rc_fidx.set(rc_fidx, s); d_rc_rptr.set(rc_rptr, s); d_rc_ridx.set(rc_ridx, s);
+        d_rS.set(rS, s); d_rcc.set(rcc, s); d_rrc.set(rrc, s); d_rstop.set(rstop, s); d_rstore.set(rstore, s);
+        d_ext_q.set(q_all, s); d_rcv_qbase.set(rcv_qbase, s); d_rcv_stride.set(rcv_stride, s);
+        d_ext_ov.set(std::vector<float>((size_t)std::max(Xt, 1), 0.0f), s);
+        d_xc_cells.set(xc_cells, s); d_xc_ptr.set(xc_ptr, s); d_xc_idx.set(xc_idx, s);
+        seg_nodes.set(nodes, order, s); seg_links.set(links, order, s); seg_icl.set(icl, order, s);
+        seg_rcl.set(rcl, order, s); seg_xcl.set(xcl, order, s);
+        d_part_inlet.alloc(2 * (size_t)std::max(icl.blocks, 1));
+        d_part_node.alloc(3 * (size_t)std::max(nodes.blocks, 1));
+        d_part_ret.alloc(2 * (size_t)std::max(rcl.blocks, 1));
+        d_part_ext.alloc((size_t)std::max(xcl.blocks, 1));
+        n_receivers = Xt;
+        // device view
+        fh::DrainDev<real>& d = dd;
+        std::memset(&d, 0, sizeof(d));
+        d.N = NT; d.I = It; d.Ci = icl.total; d.L = LT; d.S = St; d.Cr = rcl.total; d.R = Rt; d.Rcv = Xt;
+        d.Cx = xcl.total;
+        d.seg_node = seg_nodes.seg(); d.seg_link = seg_links.seg(); d.seg_icell = seg_icl.seg();
+        d.seg_ret = seg_rcl.seg(); d.seg_ext = seg_xcl.seg();
+        d.inv = d_inv.p; d.As = d_As.p; d.dmax = d_dmax.p; d.o_area = d_oa.p; d.o_coef = d_oc.p; d.o_tw = d_otw.p;
+        d.o_cap = d_ocap.p; d.nsc = d_nsc.p; d.V = d_V.p; d.sur_factor = d_sfac.p; d.overflow = d_over.p;
+        d.fallback = d_fb.p; d.wsum = d_wsum.p; d.explicit_ = d_expl.p;
+        d.ni_ptr = d_ni_ptr.p; d.ni_idx = d_ni_idx.p; d.nl_ptr = d_nl_ptr.p; d.nl_ent = d_nl_ent.p;
+        d.ne_ptr = d_ne_ptr.p; d.ne_idx = d_ne_idx.p;
+        d.icell = d_icell.p; d.inode = d_inode.p; d.capq = d_capq.p; d.surq = d_surq.p; d.sur = d_sur.p;
+        d.cap = d_cap.p; d.ic_cells = d_ic_cells.p; d.ic_ptr = d_ic_ptr.p; d.ic_idx = d_ic_idx.p;
+        d.from = d_from.p; d.to = d_to.p; d.geom = d_geom.p; d.len = d_llen.p; d.man = d_man.p; d.lcap = d_lcap.p;
+        d.linv = d_linv.p; d.D = d_D.p; d.minor = d_minor.p; d.rw = d_rw.p; d.rh = d_rh.p; d.Q = d_Q.p;
+        d.req = d_req.p; d.snode = d_snode.p; d.sw = d_sw.p; d.snorm = d_snorm.p;
+        d.rc_cells = d_rc_cells.p; d.rc_eptr = d_rc_eptr.p; d.rc_eidx = d_rc_eidx.p; d.rc_fptr = d_rc_fptr.p;
+        d.rc_fidx = d_rc_fidx.p; d.rc_rptr = d_rc_rptr.p; d.rc_ridx = d_rc_ridx.p;
+        d.rS = d_rS.p; d.rcc = d_rcc.p; d.rrc = d_rrc.p; d.rstop = d_rstop.p; d.rstore = d_rstore.p;
+        d.ext_q = d_ext_q.p; d.rcv_qbase = d_rcv_qbase.p; d.rcv_stride = d_rcv_stride.p;
+        d.ext_q_override = d_ext_ov.p; d.use_override = 0;
+        d.xc_cells = d_xc_cells.p; d.xc_ptr = d_xc_ptr.p; d.xc_idx = d_xc_idx.p;
     }
 
     // ------------------------------------------------------------------ stepping
-    void launch_step() {
-        fh::k_begin<real><<<g_begin, 256, 0, stream>>>(ctl, kp, d_state.p, d_lc.p, (int)C, d_part_begin.p,
-                                                      d_rain_s.p, d_rain_e.p, d_rain_r.p, (int)n_rain,
-                                                      drain.ext_times_ptr(), (int)n_ext_times);
-        if (E > 0)
-            fh::k_flux<real><<<g_edge, bs, 0, stream>>>(ctl, kp, (int)E_int, (int)E, d_el.p, d_er.p, d_nx.p,
-                                                       d_ny.p, d_len.p, d_state.p, d_rec.p, d_recR.p,
-                                                       d_part_flux.p);
+    fh::BeginTables begin_tables() const {
+        return fh::BeginTables{seg_begin.seg(), d_cell0.p, d_cell1.p, d_rain_off.p, d_rain_n.p, d_ext_off.p, d_ext_n.p};
+    }
+
+    void launch_begin() {
+        fh::k_begin<real><<<begin_l.blocks, 256, 0, stream>>>(ctl, kp, d_state.p, d_lc.p, begin_tables(),
+                                                              d_part_begin.p, d_rain_s.p, d_rain_e.p, d_rain_r.p,
+                                                              d_ext_t.p);
+    }
+    void launch_flux() {
+        if (edges.blocks == 0) return;
+        fh::EdgeSeg es{seg_edges.seg(), d_gblk0.p, d_gnblk.p};
+#define FH_FLUX(EPT_)                                                                                  \
+        fh::k_flux<real, EPT_><<<edges.blocks, bs, 0, stream>>>(ctl, kp, es, d_el.p, d_er.p, d_nx.p, d_ny.p, \
+                                                                d_len.p, d_state.p, d_rec.p, d_recR.p, d_part_flux.p)
+        if (ept == 4) FH_FLUX(4);
+        else if (ept == 2) FH_FLUX(2);
+        else FH_FLUX(1);
+#undef FH_FLUX
+    }
+    void launch_update() {
         if (P.compensated)
-            fh::k_update<real, true><<<g_cell, bs, 0, stream>>>(ctl, kp, (int)C, d_ce0.p, d_ce1.p, d_ce2.p,
-                                                               d_rec.p, d_recR.p, d_state.p, d_comp.p, d_wet.p,
-                                                               d_area.p, d_n2.p, d_rainm.p, d_f0.p, d_fc.p,
-                                                               d_k.p, d_part_upd.p);
+            fh::k_update<real, true><<<cells.blocks, bs, 0, stream>>>(ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p,
+                                                                     d_ce2.p, d_rec.p, d_recR.p, d_state.p, d_comp.p,
+                                                                     d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p,
+                                                                     d_fc.p, d_k.p, d_part_upd.p);
         else
-            fh::k_update<real, false><<<g_cell, bs, 0, stream>>>(ctl, kp, (int)C, d_ce0.p, d_ce1.p, d_ce2.p,
-                                                                d_rec.p, d_recR.p, d_state.p, d_comp.p, d_wet.p,
-                                                                d_area.p, d_n2.p, d_rainm.p, d_f0.p, d_fc.p,
-                                                                d_k.p, d_part_upd.p);
-        drain.launch(*this);
+            fh::k_update<real, false><<<cells.blocks, bs, 0, stream>>>(ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p,
+                                                                      d_ce2.p, d_rec.p, d_recR.p, d_state.p, d_comp.p,
+                                                                      d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p,
+                                                                      d_fc.p, d_k.p, d_part_upd.p);
+    }
+
+    template <class Timed>
+    void launch_drainage(Timed&& timed) {
+        const hipStream_t s = stream;
+        const double rb = sizeof(real);
+        const fh::DrainDev<real> d = dd;
+        if (xcl.blocks > 0)
+            timed("external", (double)d.Rcv * 12.0 + d.Cx * 16.0, [&] {
+                fh::k_external<real><<<xcl.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p, d_area.p, d_part_ext.p);
+            });
+        if (nodes.blocks == 0) return;
+        if (icl.blocks > 0) {
+            timed("inlet_node_factor", (double)d.N * (8 + 3 * rb) + (double)d.I * (12 + 5 * rb), [&] {
+                fh::k_inlet_node_factor<real><<<nodes.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p);
+            });
+            timed("inlet_cell", (double)d.Ci * (12 + 4 * rb) + (double)d.I * (8 + 5 * rb), [&] {
+                fh::k_inlet_cell<real><<<icl.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p, d_area.p, d_part_inlet.p);
+            });
+            timed("inlet_node_apply", (double)d.N * (8 + 2 * rb) + (double)d.I * (4 + 2 * rb), [&] {
+                fh::k_inlet_node_apply<real><<<nodes.blocks, 256, 0, s>>>(ctl, d);
+            });
+        }
+        if (links.blocks > 0) {
+            timed("link", (double)d.L * (12 + 11 * rb) + (double)d.N * 3 * rb, [&] {
+                fh::k_link<real><<<links.blocks, 256, 0, s>>>(ctl, kp, d);
+            });
+            timed("node_factor", (double)d.N * (4 + 2 * rb) + (double)d.L * (8 + rb), [&] {
+                fh::k_node_factor<real><<<nodes.blocks, 256, 0, s>>>(ctl, d);
+            });
+        }
+        timed("node_update", (double)d.N * (16 + 14 * rb) + (double)d.L * (16 + 3 * rb), [&] {
+            fh::k_node_update<real><<<nodes.blocks, 256, 0, s>>>(ctl, kp, d, d_part_node.p);
+        });
+        if (rcl.blocks > 0)
+            timed("return", (double)d.Cr * (16 + 3 * rb) + (double)d.N * 2 * rb, [&] {
+                fh::k_return<real><<<rcl.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p, d_area.p, d_part_ret.p);
+            });
+    }
+
+    void launch_step() {
+        launch_begin();
+        launch_flux();
+        launch_update();
+        launch_drainage([](const char*, double, auto&& fn) { fn(); });
     }
 
     void drop_graph() {
@@ -478,77 +1050,145 @@ public:
         }
     }
 
+    void finalize_pending() {
+        fh::k_finalize_wet<real><<<std::max(cells.blocks, 1), bs, 0, stream>>>(ctl, kp, seg_cells.seg(), d_state.p,
+                                                                               d_wet.p);
+        fh::k_clear_pending<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M);
+    }
+
     bool check_finite() {
-        fh::k_finite<real><<<g_fin(), 256, 0, stream>>>(ctl, (int)C, d_state.p, (int)N, drain.volume_ptr(),
-                                                        d_part_fin.p);
+        const int g = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (cells.total + 255) / 256));
+        fh::k_finite<real><<<g, 256, 0, stream>>>(ctl, cells.total, d_state.p, nodes.total, d_V.p, d_part_fin.p);
         pull_ctl();
-        return ctl_h->nonfinite == 0;
+        return ctl_h[0].nonfinite == 0;
+    }
+
+    bool any_running(double target, int64_t max_steps) const {
+        for (int m = 0; m < M; ++m)
+            if (ctl_h[m].t < target && ctl_h[m].interval_step < max_steps) return true;
+        return false;
     }
 
     int advance_to(double target, double final_t, int64_t max_steps) override {
-        need_surface();
+        ensure_packed();
         if (max_steps <= 0) throw ValueError{"max_steps must be positive"};
-        fh::k_set_target<<<1, 1, 0, stream>>>(ctl, target, final_t, max_steps);
+        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, target, final_t, max_steps);
         const int K = std::max(1, P.steps_per_launch);
         pull_ctl();
-        int64_t done = 0;
-        while (ctl_h->t < target && ctl_h->interval_step < max_steps) {
-            // launch several chunks when far from the target (extra steps are no-ops)
-            const double dt_est = std::max(ctl_h->dt > 0 ? ctl_h->dt : 1e-3, 1e-6);
-            const double remaining = (target - ctl_h->t) / dt_est;
-            int chunks = (int)std::min<double>(8.0, std::max(1.0, std::floor(remaining / K)));
-            const int64_t budget_left = max_steps - ctl_h->interval_step;
+        while (any_running(target, max_steps)) {
+            // several chunks when every member is far from the target (extra steps are no-ops)
+            double min_rem = 1e300;
+            int64_t budget_left = 0;
+            for (int m = 0; m < M; ++m) {
+                if (!(ctl_h[m].t < target && ctl_h[m].interval_step < max_steps)) continue;
+                const double dt_est = std::max(ctl_h[m].dt > 0 ? ctl_h[m].dt : 1e-3, 1e-6);
+                min_rem = std::min(min_rem, (target - ctl_h[m].t) / dt_est);
+                budget_left = std::max<int64_t>(budget_left, max_steps - ctl_h[m].interval_step);
+            }
+            int chunks = (int)std::min<double>(8.0, std::max(1.0, std::floor(min_rem / K)));
             chunks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, (budget_left + K - 1) / K));
             for (int i = 0; i < chunks; ++i) launch_chunk(K);
             pull_ctl();
-            done += (int64_t)chunks * K;
         }
         finalize_pending();
         const bool finite = check_finite();
         if (!finite) return FH_ERR_SIMULATION;
-        if (ctl_h->t < target) return FH_ERR_BUDGET;
+        for (int m = 0; m < M; ++m)
+            if (ctl_h[m].t < target) return FH_ERR_BUDGET;
         return FH_OK;
     }
 
     void step_once(double dt, double rain, const float* q, double* diag) override {
-        need_surface();
+        ensure_packed();
         if (!(dt > 0.0)) throw ValueError{"timestep_s must be positive."};
-        drain.set_external_override(*this, q);
-        fh::k_set_target<<<1, 1, 0, stream>>>(ctl, INFINITY, INFINITY, 1);
-        fh::k_set_override<<<1, 1, 0, stream>>>(ctl, 1, dt, rain);
+        if (n_receivers > 0) {
+            std::vector<float> ov((size_t)n_receivers, 0.0f);
+            // the override table holds the selected member's receivers (single-member use)
+            if (q) {
+                const int xb = 0;
+                for (int64_t j = 0; j < cm().Rcv; ++j) ov[xb + j] = q[j];
+            }
+            d_ext_ov.set(ov, stream);
+            dd.use_override = 1;
+        }
+        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, INFINITY, INFINITY, 1);
+        fh::k_set_override<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, 1, dt, rain);
         launch_step();
-        fh::k_set_override<<<1, 1, 0, stream>>>(ctl, 0, 0.0, 0.0);
+        fh::k_set_override<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, 0, 0.0, 0.0);
         finalize_pending();
+        dd.use_override = 0;
         pull_ctl();
-        drain.clear_external_override(*this);
-        // restore CFL cadence for the next policy step
-        ctl_h->eval_next = (P.fixed_dt <= 0.0 && (ctl_h->step % kp.interval) == 0) ? 1 : 0;
+        for (int m = 0; m < M; ++m)
+            ctl_h[m].eval_next = (P.fixed_dt <= 0.0 && (ctl_h[m].step % kp.interval) == 0) ? 1 : 0;
         push_ctl();
         if (diag)
-            for (int k = 0; k < fh::kLedgerTerms; ++k) diag[k] = ctl_h->diag[k];
+            for (int k = 0; k < fh::kLedgerTerms; ++k) diag[k] = ctl_h[cur].diag[k];
     }
 
     double stable_timestep() override {
-        need_surface();
-        // run the device reduction of k_begin on an inactive step (target = -inf):
-        // it records the CFL minimum and leaves the clock untouched
+        ensure_packed();
+        // run k_begin's reduction on inactive steps (target = -inf): the clock is untouched
         pull_ctl();
-        fh::Ctl saved = *ctl_h;
-        ctl_h->eval_next = 1;
-        ctl_h->override_dt = 0;
-        ctl_h->target = -INFINITY;
+        std::vector<fh::Ctl> saved(ctl_h, ctl_h + M);
+        for (int m = 0; m < M; ++m) {
+            ctl_h[m].eval_next = 1;
+            ctl_h[m].override_dt = 0;
+            ctl_h[m].target = -INFINITY;
+        }
         push_ctl();
-        fh::k_begin<real><<<g_begin, 256, 0, stream>>>(ctl, kp, d_state.p, d_lc.p, (int)C, d_part_begin.p,
-                                                      d_rain_s.p, d_rain_e.p, d_rain_r.p, 0, nullptr, 0);
+        launch_begin();
         pull_ctl();
-        const double r = ctl_h->cfl_min;
-        *ctl_h = saved;
-        ctl_h->cfl_min = r;
+        const double r = ctl_h[cur].cfl_min;
+        for (int m = 0; m < M; ++m) {
+            const double cm_ = ctl_h[m].cfl_min;
+            ctl_h[m] = saved[m];
+            ctl_h[m].cfl_min = cm_;
+        }
         push_ctl();
         return r;
     }
 
-    std::vector<KernelTiming> profile(int steps) override;
+    std::vector<KernelTiming> profile(int steps) override {
+        ensure_packed();
+        std::vector<KernelTiming> out;
+        auto timed = [&](const char* name, double bytes, auto&& fn) {
+            hipEvent_t a, b;
+            FH_CHECK(hipEventCreate(&a));
+            FH_CHECK(hipEventCreate(&b));
+            FH_CHECK(hipEventRecord(a, stream));
+            fn();
+            FH_CHECK(hipEventRecord(b, stream));
+            FH_CHECK(hipEventSynchronize(b));
+            float ms = 0.f;
+            FH_CHECK(hipEventElapsedTime(&ms, a, b));
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            auto it = std::find_if(out.begin(), out.end(), [&](const KernelTiming& k) { return k.name == name; });
+            if (it == out.end()) {
+                out.push_back(KernelTiming{name, 0.0, bytes, 0});
+                it = out.end() - 1;
+            }
+            it->total_ms += ms;
+            it->launches += 1;
+        };
+        double Ct = 0, Et = 0;
+        for (const auto& m : mem) { Ct += (double)m.C; Et += (double)m.E; }
+        const double R4 = sizeof(real), I4 = 4.0;
+        // bytes each kernel must move (each required element once per step)
+        const double flux_bytes = Et * (2 * I4 + 3 * R4) + Ct * 4 * R4 + Et * 5 * R4;
+        const double upd_bytes = Et * 5 * R4 + Ct * (3 * I4 + 8 * R4 + 2 * R4 + 4 * R4 + 2 * R4) +
+                                 (P.compensated ? Ct * 8 * R4 : 0.0);
+        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, INFINITY, INFINITY, (int64_t)steps + 8);
+        for (int i = 0; i < steps; ++i) {
+            timed("begin", Ct * 4 * R4 / kp.interval, [&] { launch_begin(); });
+            timed("flux", flux_bytes, [&] { launch_flux(); });
+            timed("update", upd_bytes, [&] { launch_update(); });
+            launch_drainage(timed);
+        }
+        finalize_pending();
+        FH_CHECK(hipStreamSynchronize(stream));
+        return out;
+    }
 
     int64_t device_bytes() const override {
         int64_t b = 0;
@@ -556,98 +1196,49 @@ public:
         b += d_rec.bytes() + d_recR.bytes() + d_state.bytes() + d_wet.bytes() + d_comp.bytes();
         b += d_area.bytes() + d_n2.bytes() + d_lc.bytes() + d_rainm.bytes() + d_f0.bytes() + d_fc.bytes() + d_k.bytes();
         b += d_ce0.bytes() + d_ce1.bytes() + d_ce2.bytes();
-        b += drain.device_bytes();
+        b += d_inv.bytes() * 12 + d_ni_idx.bytes() + d_nl_ent.bytes() + d_icell.bytes() * 6 + d_from.bytes() * 16;
         return b;
-    }
-
-    void need_surface() const {
-        if (!surface_ready) throw ValueError{"fh_upload_surface must be called first"};
     }
 
     // ------------------------------------------------------------------ members
     fh_params P;
+    int M = 1;
+    int cur = 0;
+    std::vector<Member<real>> mem;
+    std::vector<int> order, member_pos, rrow_base;
     fh::KP<real> kp{};
     hipStream_t stream = nullptr;
-    fh::Ctl* ctl = nullptr;
-    fh::Ctl* ctl_h = nullptr;
-    int bs = 256;
-    int g_edge = 1, g_cell = 1, g_begin = 1;
-    int64_t C = 0, E = 0, E_int = 0, N = 0, I = 0, L = 0, R = 0, n_rain = 0, n_ext_times = 0;
-    bool surface_ready = false;
-    std::vector<real> host_bed;
-    std::vector<uint8_t> host_active;
-    DevArray<int32_t> d_el, d_er, d_ce0, d_ce1, d_ce2;
+    fh::Ctl* ctl = nullptr;     // device, [M] indexed by member id
+    fh::Ctl* ctl_h = nullptr;   // pinned host mirror
+    int bs = 256, ept = 1;
+    bool packed = false, dirty = true, any_drainage = false;
+    int n_receivers = 0;
+    Layout cells, edges, nodes, links, icl, rcl, xcl, begin_l;
+    DevSeg seg_cells, seg_edges, seg_begin, seg_nodes, seg_links, seg_icl, seg_rcl, seg_xcl;
+    DevArray<int> d_gblk0, d_gnblk, d_cell0, d_cell1, d_rain_off, d_rain_n, d_ext_off, d_ext_n;
+    DevArray<int> d_el, d_er, d_ce0, d_ce1, d_ce2;
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
     DevArray<V> d_rec, d_state, d_comp;
-    DevArray<double> d_part_begin, d_part_flux, d_part_upd, d_part_fin;
-    DevArray<float> d_rain_s, d_rain_e, d_rain_r;
-    fh::Drainage<real> drain;
+    DevArray<double> d_part_begin, d_part_flux, d_part_upd, d_part_fin, d_part_inlet, d_part_node, d_part_ret,
+        d_part_ext;
+    DevArray<float> d_rain_s, d_rain_e, d_rain_r, d_ext_t, d_ext_q, d_ext_ov;
+    // drainage
+    DevArray<real> d_inv, d_As, d_dmax, d_oa, d_oc, d_otw, d_ocap, d_V, d_sfac, d_over, d_fb, d_wsum;
+    DevArray<int> d_nsc, d_expl, d_ni_ptr, d_ni_idx, d_nl_ptr, d_nl_ent, d_ne_ptr, d_ne_idx;
+    DevArray<int> d_icell, d_inode, d_ic_cells, d_ic_ptr, d_ic_idx;
+    DevArray<real> d_capq, d_surq, d_sur, d_cap;
+    DevArray<int> d_from, d_to, d_geom;
+    DevArray<real> d_llen, d_man, d_lcap, d_linv, d_D, d_minor, d_rw, d_rh, d_Q, d_req;
+    DevArray<int> d_snode, d_snorm;
+    DevArray<real> d_sw;
+    DevArray<int> d_rc_cells, d_rc_eptr, d_rc_eidx, d_rc_fptr, d_rc_fidx, d_rc_rptr, d_rc_ridx;
+    DevArray<real> d_rS, d_rcc, d_rrc, d_rstop, d_rstore;
+    DevArray<int> d_rcv_qbase, d_rcv_stride, d_xc_cells, d_xc_ptr, d_xc_idx;
+    fh::DrainDev<real> dd{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
 };
-
-template <typename real>
-std::vector<KernelTiming> Engine<real>::profile(int steps) {
-    need_surface();
-    // time each kernel of the step sequence with events on the context stream
-    std::vector<KernelTiming> out;
-    auto timed = [&](const char* name, double bytes, auto&& fn) {
-        hipEvent_t a, b;
-        FH_CHECK(hipEventCreate(&a));
-        FH_CHECK(hipEventCreate(&b));
-        FH_CHECK(hipEventRecord(a, stream));
-        fn();
-        FH_CHECK(hipEventRecord(b, stream));
-        FH_CHECK(hipEventSynchronize(b));
-        float ms = 0.f;
-        FH_CHECK(hipEventElapsedTime(&ms, a, b));
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
-        auto it = std::find_if(out.begin(), out.end(), [&](const KernelTiming& k) { return k.name == name; });
-        if (it == out.end()) {
-            out.push_back(KernelTiming{name, 0.0, bytes, 0});
-            it = out.end() - 1;
-        }
-        it->total_ms += ms;
-        it->launches += 1;
-    };
-    const double R4 = sizeof(real), I4 = 4.0;
-    // algorithmic bytes (each required element once per step)
-    const double flux_bytes = (double)E * (2 * I4 + 3 * R4) + (double)C * 5 * R4 + (double)E * 5 * R4;
-    const double upd_bytes = (double)E * 5 * R4 + (double)C * (3 * I4 + 6 * R4 + 3 * R4 + 4 * R4 + 2 * R4) +
-                             (P.compensated ? (double)C * 6 * R4 : 0.0);
-    fh::k_set_target<<<1, 1, 0, stream>>>(ctl, INFINITY, INFINITY, (int64_t)steps + 8);
-    for (int i = 0; i < steps; ++i) {
-        timed("begin", (double)C * 4 * R4 / kp.interval, [&] {
-            fh::k_begin<real><<<g_begin, 256, 0, stream>>>(ctl, kp, d_state.p, d_lc.p, (int)C, d_part_begin.p,
-                                                          d_rain_s.p, d_rain_e.p, d_rain_r.p, (int)n_rain,
-                                                          drain.ext_times_ptr(), (int)n_ext_times);
-        });
-        if (E > 0)
-            timed("flux", flux_bytes, [&] {
-                fh::k_flux<real><<<g_edge, bs, 0, stream>>>(ctl, kp, (int)E_int, (int)E, d_el.p, d_er.p, d_nx.p,
-                                                           d_ny.p, d_len.p, d_state.p, d_rec.p, d_recR.p,
-                                                           d_part_flux.p);
-            });
-        timed("update", upd_bytes, [&] {
-            if (P.compensated)
-                fh::k_update<real, true><<<g_cell, bs, 0, stream>>>(ctl, kp, (int)C, d_ce0.p, d_ce1.p, d_ce2.p,
-                                                                   d_rec.p, d_recR.p, d_state.p, d_comp.p,
-                                                                   d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p,
-                                                                   d_fc.p, d_k.p, d_part_upd.p);
-            else
-                fh::k_update<real, false><<<g_cell, bs, 0, stream>>>(ctl, kp, (int)C, d_ce0.p, d_ce1.p, d_ce2.p,
-                                                                    d_rec.p, d_recR.p, d_state.p, d_comp.p,
-                                                                    d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p,
-                                                                    d_fc.p, d_k.p, d_part_upd.p);
-        });
-        drain.profile(*this, timed);
-    }
-    finalize_pending();
-    FH_CHECK(hipStreamSynchronize(stream));
-    return out;
-}
 
 struct Ctx {
     std::unique_ptr<EngineBase> eng;
@@ -699,9 +1290,11 @@ void fh_default_params(fh_params* p) {
     p->block_size = 256;
     p->xcd_remap = 1;
     p->device = 0;
+    p->edges_per_thread = 1;
+    p->pack_order = 0;
 }
 
-const char* fh_version(void) { return "floodhip 0.1.0 (gfx950)"; }
+const char* fh_version(void) { return "floodhip 0.2.0 (gfx950)"; }
 
 int fh_device_count(void) {
     int n = 0;
@@ -711,7 +1304,7 @@ int fh_device_count(void) {
 
 const char* fh_global_error(void) { return g_error.c_str(); }
 
-fh_ctx* fh_create(const fh_params* params) {
+fh_ctx* fh_create_batch(const fh_params* params, int32_t members) {
     fh_params p;
     if (params) p = *params; else fh_default_params(&p);
     int n = 0;
@@ -719,18 +1312,13 @@ fh_ctx* fh_create(const fh_params* params) {
         g_error = "no HIP device is visible (BackendUnavailableError)";
         return nullptr;
     }
-    if (p.device < 0 || p.device >= n) {
-        g_error = "device ordinal out of range";
-        return nullptr;
-    }
-    if (p.precision != 32 && p.precision != 64) {
-        g_error = "precision must be 32 or 64";
-        return nullptr;
-    }
+    if (p.device < 0 || p.device >= n) { g_error = "device ordinal out of range"; return nullptr; }
+    if (p.precision != 32 && p.precision != 64) { g_error = "precision must be 32 or 64"; return nullptr; }
+    if (members < 1 || members > 65536) { g_error = "members must be in [1, 65536]"; return nullptr; }
     try {
         auto* ctx = new fh_ctx();
-        if (p.precision == 64) ctx->c.eng.reset(new Engine<double>(p));
-        else ctx->c.eng.reset(new Engine<float>(p));
+        if (p.precision == 64) ctx->c.eng.reset(new Engine<double>(p, members));
+        else ctx->c.eng.reset(new Engine<float>(p, members));
         return ctx;
     } catch (const HipError& h) {
         g_error = std::string("HIP error ") + hipGetErrorString(h.code) + " in " + h.where;
@@ -740,9 +1328,18 @@ fh_ctx* fh_create(const fh_params* params) {
     return nullptr;
 }
 
+fh_ctx* fh_create(const fh_params* params) { return fh_create_batch(params, 1); }
+
 void fh_destroy(fh_ctx* ctx) { delete ctx; }
 
 const char* fh_last_error(const fh_ctx* ctx) { return ctx ? ctx->c.err.c_str() : g_error.c_str(); }
+
+int fh_select_member(fh_ctx* ctx, int32_t member) {
+    FH_GUARD(ctx, ctx->c.eng->select(member));
+    return FH_OK;
+}
+
+int32_t fh_member_count(const fh_ctx* ctx) { return ctx ? ctx->c.eng->members() : 0; }
 
 int fh_upload_surface(fh_ctx* ctx, int64_t n_cells, int64_t n_edges, const float* bed, const float* manning_n,
                       const float* cell_area, const float* lc, const int32_t* edge_left,

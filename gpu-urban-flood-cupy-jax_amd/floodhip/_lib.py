@@ -25,6 +25,7 @@ class FhParams(C.Structure):
         ("fixed_dt", C.c_double), ("cfl_interval", C.c_int32), ("compensated", C.c_int32),
         ("precision", C.c_int32), ("use_graph", C.c_int32), ("steps_per_launch", C.c_int32),
         ("block_size", C.c_int32), ("xcd_remap", C.c_int32), ("device", C.c_int32),
+        ("edges_per_thread", C.c_int32), ("pack_order", C.c_int32),
     ]
 
 
@@ -47,6 +48,9 @@ SIGNATURES = {
     "fh_version": (C.c_char_p, []),
     "fh_device_count": (C.c_int, []),
     "fh_create": (_ctx, [C.POINTER(FhParams)]),
+    "fh_create_batch": (_ctx, [C.POINTER(FhParams), C.c_int32]),
+    "fh_select_member": (C.c_int, [_ctx, C.c_int32]),
+    "fh_member_count": (C.c_int32, [_ctx]),
     "fh_destroy": (None, [_ctx]),
     "fh_last_error": (C.c_char_p, [_ctx]),
     "fh_global_error": (C.c_char_p, []),

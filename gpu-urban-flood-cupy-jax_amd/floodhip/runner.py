@@ -1,0 +1,151 @@
+"""Output-time driver mirroring ``run_jax_v1_parity``
+(``hybrid_flood/operational/parity_runner.py:245-437``) on the MI355X backend.
+
+Targets are ``[1e-6, 3600, 7200, ..., duration]`` (the first snapshot follows the
+first step, :297-299); after each target the clock, finiteness and the step
+budget are checked (:311-322) and h, speed (h > 1e-4) and node volume are
+snapshotted.  The report carries the reference's field names, ledger terms
+(``<term>_m3``) and the balance residuals (:333-345).  ``publish`` writes the
+reference's NPZ + run_report.json layout atomically (:401-436).
+"""
+
+from __future__ import annotations
+
+import json
+import shutil
+import uuid
+from pathlib import Path
+from time import perf_counter
+
+import numpy as np
+
+from .errors import ConfigurationError, SimulationError
+from .solver import HipParitySolver
+from .types import LEDGER_FIELDS, LoadedParityInputs
+
+
+def _speed(st) -> np.ndarray:
+    """parity_runner.py:237-242."""
+    speed = np.zeros_like(st.h, dtype=np.float32)
+    wet = st.h > 1.0e-4
+    speed[wet] = np.sqrt(st.hu[wet] ** 2 + st.hv[wet] ** 2) / st.h[wet]
+    return speed
+
+
+def output_targets(duration_s: float, interval_s: float = 3600.0) -> np.ndarray:
+    requested = np.arange(0.0, duration_s + 1.0, interval_s, dtype=np.float32)
+    if requested[-1] < duration_s:
+        requested = np.append(requested, np.float32(duration_s))
+    return requested
+
+
+def run_hip_v1_parity(inputs: LoadedParityInputs, *, duration_s: float, max_steps_per_hour: int = 20_000,
+                      snapshot_interval_s: float = 3600.0, solver: HipParitySolver | None = None,
+                      keep_final_state: bool = True, **solver_opts) -> dict:
+    if duration_s <= 0:
+        raise ConfigurationError("Parity duration must be positive.")
+    s = solver or HipParitySolver(inputs.surface, inputs.coupling, inputs.pipe, inputs.recharge,
+                                  inputs.forcing, inputs.initial_state, **solver_opts)
+    area = np.asarray(inputs.surface.cell_area, np.float64)
+    initial_surface = float(np.sum(np.asarray(inputs.initial_state.h, np.float64) * area))
+    initial_node = float(np.sum(np.asarray(inputs.initial_state.node_volume, np.float64)))
+    requested = output_targets(duration_s, snapshot_interval_s)
+    depth, speed, nodes, times, steps_at, chunk_rt = [], [], [], [], [], []
+    run_started = perf_counter()
+    ledger = None
+    st = None
+    for i, req in enumerate(requested):
+        target = np.float32(1.0e-6) if i == 0 else req
+        t0 = perf_counter()
+        clock, ledger = s.advance_to(float(target), float(duration_s), max_steps_per_hour,
+                                     raise_on_budget=False)
+        chunk_rt.append(perf_counter() - t0)
+        reached = clock["time_s"]
+        if reached + 1.0e-3 < float(target):
+            raise SimulationError(f"max_steps_per_hour={max_steps_per_hour} reached only {reached:.3f}s "
+                                  f"of target {float(target):.3f}s.")
+        st = s.get_state()
+        if not np.isfinite(st.h).all():
+            raise SimulationError(f"Non-finite surface depth at t={reached:.3f}s.")
+        if not np.isfinite(st.node_volume).all():
+            raise SimulationError(f"Non-finite drainage storage at t={reached:.3f}s.")
+        depth.append(st.h.astype(np.float32))
+        speed.append(_speed(st))
+        nodes.append(st.node_volume.astype(np.float32))
+        times.append(reached)
+        steps_at.append(int(clock["global_step"]))
+    runtime = perf_counter() - run_started
+    depth_a = np.stack(depth)
+    node_a = np.stack(nodes)
+    led = {name: float(v) for name, v in zip(LEDGER_FIELDS, ledger)}
+    final_surface = float(np.sum(depth_a[-1].astype(np.float64) * area))
+    final_node = float(np.sum(node_a[-1].astype(np.float64)))
+    final_recharge = float(np.sum(np.asarray(st.recharge_storage, np.float64)))
+    lhs = initial_surface + initial_node + led["rainfall_input"] + led["external_boundary_inflow"]
+    rhs_v1 = (final_surface + final_node + led["infiltration_loss"] + led["outfall_discharge"]
+              + led["unreturned_overflow"] + led["recharge_loss"] - led["boundary_net_flux"])
+    rhs_complete = rhs_v1 + final_recharge
+    clock = s.clock()
+    report = {
+        "implementation": "floodhip_mi355x", "backend": "hip_gfx950", "is_gpu": True,
+        "surface_step_mode": "full_swe", "drainage_mode": "v1_full_dynamic_network",
+        "duration_s": float(duration_s), "runtime_s": runtime, "chunk_runtimes_s": chunk_rt,
+        "step_count": int(clock["global_step"]), "steps_at_snapshot": steps_at,
+        "cfl_eval_count": int(clock["cfl_evaluations"]),
+        "snapshot_count": len(times), "max_depth_m": float(np.max(depth_a)),
+        "drainage_node_count": int(node_a.shape[1]),
+        "initial_surface_volume_m3": initial_surface, "initial_node_volume_m3": initial_node,
+        "final_surface_volume_m3": final_surface, "final_node_volume_m3": final_node,
+        "final_recharge_storage_m3": final_recharge,
+        **{f"{k}_m3": v for k, v in led.items()},
+        "v1_compatible_mass_residual_m3": lhs - rhs_v1,
+        "v1_compatible_mass_residual_ratio": (lhs - rhs_v1) / max(abs(lhs), 1.0),
+        "complete_mass_residual_m3": lhs - rhs_complete,
+        "complete_mass_residual_ratio": (lhs - rhs_complete) / max(abs(lhs), 1.0),
+        "validation_status": "parity_candidate_unvalidated",
+    }
+    out = {"snapshot_times_s": np.asarray(times, np.float32), "steps_at_snapshot": np.asarray(steps_at, np.int64),
+           "depth_snapshots_m": depth_a, "speed_snapshots_mps": np.stack(speed),
+           "drain_node_volume_snapshots_m3": node_a, "report": report}
+    if keep_final_state:
+        out["final_state"] = st
+    if solver is None:
+        s.close()
+    return out
+
+
+def publish(result: dict, destination, inputs: LoadedParityInputs) -> Path:
+    """Write the reference's run-directory layout atomically (parity_runner.py:401-436)."""
+    dest = Path(destination).resolve()
+    if dest.exists():
+        raise ConfigurationError(f"Parity output directory already exists: {dest}")
+    partial = dest.parent / f".{dest.name}.partial-{uuid.uuid4().hex}"
+    partial.mkdir(parents=True, exist_ok=False)
+    try:
+        depth = result["depth_snapshots_m"]
+        st = result.get("final_state")
+        arrays = dict(
+            snapshot_times_min=result["snapshot_times_s"] / 60.0,
+            snapshot_times_s=result["snapshot_times_s"], steps_at_snapshot=result["steps_at_snapshot"],
+            depth_snapshots_m=depth, speed_snapshots_mps=result["speed_snapshots_mps"],
+            drain_node_volume_snapshots_m3=result["drain_node_volume_snapshots_m3"],
+            final_depth_m=depth[-1], max_depth_m=np.max(depth, axis=0),
+            max_speed_mps=np.max(result["speed_snapshots_mps"], axis=0),
+            final_node_volume_m3=result["drain_node_volume_snapshots_m3"][-1],
+            active_surface_mask=np.asarray(inputs.surface.active, bool),
+            terrain_elevation_m=np.asarray(inputs.surface.bed, np.float32),
+        )
+        if st is not None:
+            arrays.update(final_hu_m2ps=st.hu, final_hv_m2ps=st.hv, final_link_flow_m3ps=st.link_flow)
+        if "triangles" in inputs.host:
+            arrays["triangles"] = np.asarray(inputs.host["triangles"], np.int32)
+        if "centroids" in inputs.host:
+            arrays["centroids_world"] = np.asarray(inputs.host["centroids"], np.float32)
+        np.savez_compressed(partial / "flood_simulation_outputs.npz", **arrays)
+        (partial / "run_report.json").write_text(json.dumps(result["report"], indent=2), encoding="utf-8")
+        partial.rename(dest)
+    except Exception:
+        if partial.exists():
+            shutil.rmtree(partial)
+        raise
+    return dest

@@ -77,6 +77,8 @@ class SolverOptions(NamedTuple):
     block_size: int = 256
     xcd_remap: bool = True
     device: int = 0
+    edges_per_thread: int = 1
+    pack_order: int = 0
     cfl: float = 0.85
     cfl_interval: int = 32
     reuse_factor: float = 0.8
@@ -88,10 +90,10 @@ class SolverOptions(NamedTuple):
 class HipParitySolver:
     """One device-resident scenario on one GPU (one HIP stream)."""
 
-    def __init__(self, surface: SurfaceTopology, coupling: CouplingTopology | None = None,
+    def __init__(self, surface: SurfaceTopology | None, coupling: CouplingTopology | None = None,
                  pipe: PipeTopology | None = None, recharge: RechargeTopology | None = None,
                  forcing: ParityForcing | None = None, state: ParityState | None = None,
-                 options: SolverOptions | None = None, **kw):
+                 options: SolverOptions | None = None, members: int = 1, **kw):
         self.lib = _lib.load()
         opts = (options or SolverOptions())._replace(**kw)
         self.options = opts
@@ -113,15 +115,34 @@ class HipParitySolver:
         p.max_dt = float(opts.max_dt)
         p.dt_floor = float(opts.dt_floor)
         p.max_speed = float(opts.max_speed)
-        self._ctx = self.lib.fh_create(C.byref(p))
+        p.edges_per_thread = int(opts.edges_per_thread)
+        p.pack_order = int(opts.pack_order)
+        self._ctx = self.lib.fh_create_batch(C.byref(p), int(members))
         if not self._ctx:
             msg = self.lib.fh_global_error().decode()
             raise BackendUnavailableError(f"fh_create failed: {msg}")
-        self.n_cells = int(np.asarray(surface.cell_area).shape[0])
-        self.n_nodes = 0
-        self.n_links = 0
-        self.n_recharge = 0
-        self.n_receivers = 0
+        self.members = int(members)
+        self._sizes = [dict(n_cells=0, n_nodes=0, n_links=0, n_recharge=0, n_receivers=0)
+                       for _ in range(self.members)]
+        self._member = 0
+        if surface is not None:
+            self.load_member(0, surface, coupling, pipe, recharge, forcing, state)
+
+    # per-member sizes of the selected member
+    n_cells = property(lambda self: self._sizes[self._member]["n_cells"])
+    n_nodes = property(lambda self: self._sizes[self._member]["n_nodes"])
+    n_links = property(lambda self: self._sizes[self._member]["n_links"])
+    n_recharge = property(lambda self: self._sizes[self._member]["n_recharge"])
+    n_receivers = property(lambda self: self._sizes[self._member]["n_receivers"])
+
+    def select(self, member: int):
+        """Direct upload/state/clock/ledger calls to one member of a batch."""
+        self._check(self.lib.fh_select_member(self._ctx, int(member)), "select_member")
+        self._member = int(member)
+        return self
+
+    def load_member(self, member, surface, coupling=None, pipe=None, recharge=None, forcing=None, state=None):
+        self.select(member)
         self.upload_surface(surface)
         if coupling is not None and np.asarray(coupling.node_storage_area).shape[0] > 0:
             self.upload_coupling(coupling)
@@ -178,8 +199,7 @@ class HipParitySolver:
             self._ctx, C_, E_, a.f(s.bed), a.f(s.manning_n), a.f(s.cell_area),
             a.f(s.characteristic_length), a.i(s.edge_left), a.i(s.edge_right), a.f(s.edge_nx),
             a.f(s.edge_ny), a.f(s.edge_length), a.i(s.boundary_code), a.u8(s.active)), "upload_surface")
-        self.n_cells = C_
-        self.surface = s
+        self._sizes[self._member]["n_cells"] = C_
 
     def upload_coupling(self, c: CouplingTopology):
         a = _Args()
@@ -197,7 +217,7 @@ class HipParitySolver:
             a.f(c.inlet_capture_capacity), a.f(c.inlet_surcharge_capacity), a.i(c.surcharge_cell),
             a.i(c.surcharge_node), a.f(c.surcharge_weight), int(bool(c.surcharge_weight_normalized))),
             "upload_coupling")
-        self.n_nodes = N
+        self._sizes[self._member]["n_nodes"] = N
 
     def upload_pipe(self, p: PipeTopology):
         a = _Args()
@@ -215,7 +235,7 @@ class HipParitySolver:
             a.i(p.link_geometry_code), a.f(p.link_rect_width), a.f(p.link_rect_height),
             a.f(p.outfall_area), a.f(p.outfall_coefficient), a.f(p.outfall_tailwater),
             a.f(p.outfall_capacity)), "upload_pipe")
-        self.n_links = L
+        self._sizes[self._member]["n_links"] = L
 
     def upload_recharge(self, r: RechargeTopology):
         a = _Args()
@@ -223,7 +243,7 @@ class HipParitySolver:
         self._check(self.lib.fh_upload_recharge(
             self._ctx, R, a.i(r.source_cell), a.f(r.storage_capacity), a.f(r.capture_capacity),
             a.f(r.recharge_capacity), a.f(r.stop_depth)), "upload_recharge")
-        self.n_recharge = R
+        self._sizes[self._member]["n_recharge"] = R
 
     def set_forcing(self, f: ParityForcing):
         a = _Args()
@@ -238,8 +258,7 @@ class HipParitySolver:
             a.f(f.rainfall_multiplier), a.f(f.horton_f0_mps), a.f(f.horton_fc_mps), a.f(f.horton_k_per_s),
             nt, a.f(times) if nrcv else None, nrcv, a.i(rcv) if nrcv else None,
             a.f(q) if nrcv else None), "set_forcing")
-        self.n_receivers = nrcv
-        self.forcing = f
+        self._sizes[self._member]["n_receivers"] = nrcv
 
     # ------------------------------------------------------------------ state / clock
     def set_state(self, st: ParityState):
@@ -341,6 +360,36 @@ class HipParitySolver:
 
     def device_bytes(self) -> int:
         return int(self.lib.fh_device_bytes(self._ctx))
+
+
+class HipEnsemble(HipParitySolver):
+    """A batch of independent scenarios (members) advanced together on one GPU:
+    every stage of a step is one launch over all members, each member keeping its
+    own CFL dt, clock, forcing and FP64 ledger."""
+
+    def __init__(self, members, options: SolverOptions | None = None, **kw):
+        members = list(members)
+        super().__init__(None, options=options, members=len(members), **kw)
+        for m, inp in enumerate(members):
+            self.load_member(m, inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing,
+                             inp.initial_state)
+        self.select(0)
+
+    def member_states(self):
+        out = []
+        for m in range(self.members):
+            self.select(m)
+            out.append(self.get_state())
+        self.select(0)
+        return out
+
+    def member_clocks(self):
+        out = []
+        for m in range(self.members):
+            self.select(m)
+            out.append((self.clock(), self.ledger()))
+        self.select(0)
+        return out
 
 
 # ---------------------------------------------------------------------------

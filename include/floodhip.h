@@ -2,8 +2,9 @@
  * floodhip -- C ABI of the MI355X per-timestep solver for the unstructured
  * urban flood model (surface HLL shallow water + coupled 1-D drainage).
  *
- * One context = one scenario (or one batch of scenarios) resident on one GPU,
- * bound to one HIP stream.  The library owns every device allocation; all
+ * One context = one scenario, or a batch of independent scenarios ("members",
+ * each with its own mesh, network, forcing, clock, dt and ledger), resident on
+ * one GPU and bound to one HIP stream.  The library owns every device allocation; all
  * pointers passed in are HOST pointers borrowed for the duration of the call
  * (copied to the device), all pointers passed out are host buffers the caller
  * owns.  Contexts are not thread-safe.  Arrays are float32 / int32 with the
@@ -61,9 +62,11 @@ typedef struct {
     int32_t precision;          /* 32 (float state) or 64 (double state: FP64 oracle runs) */
     int32_t use_graph;          /* 1: replay captured hipGraphs of steps_per_launch steps */
     int32_t steps_per_launch;   /* steps per graph / per host check (default 32) */
-    int32_t block_size;         /* threads per block for the edge and cell kernels (64..1024) */
+    int32_t block_size;         /* threads per block for the edge and cell kernels (64..256) */
     int32_t xcd_remap;          /* 1: map blocks so each XCD works a contiguous slice */
     int32_t device;             /* HIP device ordinal */
+    int32_t edges_per_thread;   /* 1, 2 or 4 edges per thread in the flux kernel */
+    int32_t pack_order;         /* batch member order: 0 as given, 1 largest first, 2 large/small alternating */
 } fh_params;
 
 typedef struct {
@@ -81,6 +84,15 @@ int fh_device_count(void);
 
 /* Context lifetime.  fh_create returns NULL on failure (message via fh_global_error). */
 fh_ctx* fh_create(const fh_params* params);
+/* Batched ensemble: `members` independent scenarios advanced together, every
+ * stage of a step one launch over all members (no counterpart in the
+ * reference, which has no batched mode; fh_create is fh_create_batch(p, 1)).
+ * Upload, state, clock and ledger calls apply to the member chosen with
+ * fh_select_member (default 0); fh_advance_to and fh_step advance every member,
+ * each with its own CFL dt. */
+fh_ctx* fh_create_batch(const fh_params* params, int32_t members);
+int fh_select_member(fh_ctx* ctx, int32_t member);
+int32_t fh_member_count(const fh_ctx* ctx);
 void fh_destroy(fh_ctx* ctx);
 const char* fh_last_error(const fh_ctx* ctx);
 const char* fh_global_error(void);
