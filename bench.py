@@ -65,6 +65,73 @@ def cpu_baseline(inputs, steps):
     return C * steps / el, el
 
 
+def _ensemble_rate(members, steps, warmup, opts):
+    from floodhip.solver import HipEnsemble
+    sizes = [int(m.surface.cell_area.shape[0]) for m in members]
+    with HipEnsemble(members, **opts) as ens:
+        ens.advance_to(3600.0, 3600.0, max_steps=max(warmup, 1), raise_on_budget=False)
+        before = [c["global_step"] for c, _ in ens.member_clocks()]
+        t0 = time.perf_counter()
+        ens.advance_to(3600.0, 3600.0, max_steps=steps, raise_on_budget=False)
+        el = time.perf_counter() - t0
+        after = [c["global_step"] for c, _ in ens.member_clocks()]
+        prof = ens.profile_kernels(steps=5)
+    tri = sum(n * (a - b) for n, a, b in zip(sizes, after, before))
+    return tri / el, el, sum(a - b for a, b in zip(after, before)), prof
+
+
+def batch_leg(base, args, plain_rate):
+    """16 members on the production mesh, one launch per stage per step for all of them."""
+    from floodhip import synth
+    from floodhip.autotune import autotune
+    members = synth.make_shared_mesh_ensemble(base, n_members=args.batch_members)
+    opts = dict(compensated=True, use_graph=True)
+    tuned = None
+    if not args.no_autotune:
+        best, table = autotune(members[:4], steps=32, warmup=8, log=log, compensated=True)
+        opts.update(best)
+        tuned = {"best": best, "table": [[o, r] for o, r in table], "sample_members": 4}
+    log(f"batched ensemble: {len(members)} members, options {opts}")
+    rate, el, steps, prof = _ensemble_rate(members, args.batch_steps, 20, opts)
+    # sequential plain baseline with the same state precision: one member alone, default launch options
+    seq_rate, _, _, _ = _ensemble_rate([members[0]], args.batch_steps, 20, dict(compensated=True, use_graph=True))
+    log(f"batched: {rate:.4e} tri-steps/s over {steps} member-steps in {el:.3f}s; sequential plain "
+        f"{seq_rate:.4e} ({rate / seq_rate:.2f}x)")
+    C = int(base.surface.cell_area.shape[0])
+    E = int(base.surface.edge_left.shape[0])
+    M = len(members)
+    kern = {}
+    for k in prof:
+        b = survey_bytes(k["kernel"], E * M, C * M, True)
+        kern[k["kernel"]] = {"avg_us": round(1e3 * k["avg_ms"], 3),
+                             "GBps": round(b / (k["avg_ms"] * 1e-3) / 1e9, 1) if b else None}
+    return {"workload": f"{M} members x {C} triangles (shared mesh and network, own rain totals 40-200 mm/h "
+                        f"and Horton factors), float32 compensated state, FP64 ledgers, hipGraph",
+            "members": M, "value": rate, "unit": "triangle-steps/s", "timed_member_steps": steps,
+            "seconds": el, "sequential_plain_value": seq_rate, "speedup_vs_sequential_plain": rate / seq_rate,
+            "options": opts, "autotune": tuned, "kernels": kern}
+
+
+def batch_uneven_leg(args):
+    from floodhip import synth
+    from floodhip.autotune import autotune
+    log("generating the 48-member uneven ensemble")
+    members = synth.make_uneven_ensemble()
+    sizes = [int(m.surface.cell_area.shape[0]) for m in members]
+    best, table = autotune(members, steps=16, warmup=4, log=log, compensated=True)
+    opts = dict(compensated=True, **best)
+    rate, el, steps, _ = _ensemble_rate(members, 64, 8, opts)
+    seq = []
+    for m in members[:6]:
+        r, _, _, _ = _ensemble_rate([m], 64, 8, dict(compensated=True))
+        seq.append((int(m.surface.cell_area.shape[0]), r))
+    # sequential plain: total work / total time if each member ran alone at its own rate
+    seq_rate = sum(n for n, _ in seq) / sum(n / r for n, r in seq)
+    log(f"uneven batched {rate:.4e} vs sequential plain {seq_rate:.4e} (first 6 members)")
+    return {"members": len(members), "triangles": int(sum(sizes)), "value": rate, "unit": "triangle-steps/s",
+            "options": opts, "sequential_plain_value_sampled": seq_rate, "speedup_vs_sequential_plain": rate / seq_rate}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +144,11 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--block-size", type=int, default=256)
     ap.add_argument("--no-remap", action="store_true")
+    ap.add_argument("--batch-members", type=int, default=16)
+    ap.add_argument("--batch-steps", type=int, default=200)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--uneven", action="store_true", help="also run the 48-member uneven ensemble")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -127,6 +199,11 @@ def main():
                                             / (k["avg_ms"] * 1e-3) / 1e9, 1)
                                       if survey_bytes(k["kernel"], E, C, args.compensated) else None)}
                for k in prof}
+    s.close()
+    batched = None
+    if not args.no_batch and args.batch_members > 0:
+        batched = batch_leg(inp, args, tri_steps)
+    uneven = batch_uneven_leg(args) if args.uneven else None
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU baseline: oracle, {args.cpu_steps} steps of the same workload")
@@ -157,10 +234,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "kernels": kernels,
+        "batched_shared_mesh": batched,
+        "batched_uneven": uneven,
         "cpu_baseline": cpu,
         "reference_published_tri_steps_per_s": REFERENCE_CUPY_STEPS_PER_S * C,
     }
-    s.close()
     print(json.dumps(out), flush=True)
 
 

@@ -25,6 +25,7 @@ struct DrainDev {
     const real *inv, *As, *dmax, *o_area, *o_coef, *o_tw, *o_cap;
     const int* nsc;
     real* V;
+    real* Vp;               // node volume after the inlet exchange of this step
     real *sur_factor, *overflow, *fallback, *wsum;
     const int* explicit_;  // node has valid entry-table rows
     const int *ni_ptr, *ni_idx;        // node -> inlets
@@ -212,21 +213,16 @@ __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<r
     commit_terms<2>(ctl, C_INLET, part + 2 * (size_t)blk0, nblk, terms, lds);
 }
 
-// Inlet exchange, pass C (one thread per node with inlets): V = max(V + sum(cap - sur), 0).
+// Node volume after the inlet exchange (v1_parity.py:292-298):
+// max(V + sum over the node's inlets of (capture - surcharge), 0), for every node.
 template <typename real>
-__global__ void __launch_bounds__(256) k_inlet_node_apply(const Ctl* __restrict__ ctls, DrainDev<real> d) {
-    const int mem = d.seg_node.blk_member[blockIdx.x];
-    if (!ctls[mem].active) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= d.seg_node.mem_end[mem]) return;
-    const int k0 = d.ni_ptr[n], k1 = d.ni_ptr[n + 1];
-    if (k0 == k1) return;
+__device__ __forceinline__ real post_inlet_volume(const DrainDev<real>& d, int n) {
     real acc = real(0);
-    for (int k = k0; k < k1; ++k) {
+    for (int k = d.ni_ptr[n]; k < d.ni_ptr[n + 1]; ++k) {
         const int i = d.ni_idx[k];
         acc = acc + (d.cap[i] - d.sur[i]);
     }
-    d.V[n] = vmax(d.V[n] + acc, real(0));
+    return vmax(d.V[n] + acc, real(0));
 }
 
 template <typename real>
@@ -238,20 +234,15 @@ __device__ __forceinline__ void circular_theta(real depth, real diameter, real& 
     theta = real(2) * acos(x);
 }
 
-// Pipe link (one thread per link): semi-implicit Saint-Venant momentum with
-// Manning friction and minor losses, clipped to capacity; stores the signed
-// requested transfer Q dt.
+// Pipe link momentum (v1_parity.py:324-347): semi-implicit Saint-Venant update
+// with Manning friction and minor losses, clipped to capacity; returns the
+// signed requested transfer Q dt.  Va / Vb are the end-node volumes.
 template <typename real>
-__global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d) {
-    const int mem = d.seg_link.blk_member[blockIdx.x];
-    const Ctl* ctl = ctls + mem;
-    if (!ctl->active) return;
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= d.seg_link.mem_end[mem]) return;
-    const real dt = (real)ctl->dt_real;
+__device__ __forceinline__ real link_request(const KP<real>& p, const DrainDev<real>& d, int l, real Va,
+                                             real Vb, real dt) {
     const real zero = real(0), g = p.g;
     const int a = d.from[l], b = d.to[l];
-    const real ha = node_head(d, a, d.V[a]), hb = node_head(d, b, d.V[b]);
+    const real ha = node_head(d, a, Va), hb = node_head(d, b, Vb);
     const real depth = vmax(real(0.5) * (ha + hb) - d.linv[l], zero);
     const real Draw = d.D[l];
     const real Dm = vmax(Draw, real(1.0e-12));
@@ -282,27 +273,37 @@ __global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<r
     real cand = (q0 + g * area * dt * slope) / (real(1) + dt * (friction + minor));
     const real cap = vmax(d.lcap[l], zero);
     cand = vmin(vmax(cand, -cap), cap);
-    d.req[l] = cand * dt;
+    return cand * dt;
 }
 
-// Node donor factor (one thread per node): sum of |requested| over the links
-// this node donates to (ascending link order), factor min(1, V / sum).
+// Links and donor factors (one thread per node): the node's post-inlet volume,
+// the requested transfer of every incident link (recomputed by both end nodes
+// from identical inputs; the 'from' node stores it), and the donor factor
+// min(1, V / sum of |requested| over the links this node donates to, in
+// ascending link order) -- v1_parity.py:348-352.
 template <typename real>
-__global__ void __launch_bounds__(256) k_node_factor(const Ctl* __restrict__ ctls, DrainDev<real> d) {
+__global__ void __launch_bounds__(256) k_node_links(const Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d) {
     const int mem = d.seg_node.blk_member[blockIdx.x];
-    if (!ctls[mem].active) return;
+    const Ctl* ctl = ctls + mem;
+    if (!ctl->active) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= d.seg_node.mem_end[mem]) return;
+    const real dt = (real)ctl->dt_real;
+    const real vn = post_inlet_volume(d, n);
+    d.Vp[n] = vn;
     real total = real(0);
     for (int k = d.nl_ptr[n]; k < d.nl_ptr[n + 1]; ++k) {
         const int en = d.nl_ent[k];
         const int l = en >> 1;
         const bool is_to = en & 1;
-        const real r = d.req[l];
+        const int other = is_to ? d.from[l] : d.to[l];
+        const real vo = other == n ? vn : post_inlet_volume(d, other);
+        const real r = is_to ? link_request(p, d, l, vo, vn, dt) : link_request(p, d, l, vn, vo, dt);
+        if (!is_to) d.req[l] = r;
         const bool positive = r >= real(0);
         if (positive != is_to) total = total + fabs(r);   // donor = from if positive else to
     }
-    d.sur_factor[n] = vmin(real(1), d.V[n] / vmax(total, real(1.0e-12)));
+    d.sur_factor[n] = vmin(real(1), vn / vmax(total, real(1.0e-12)));
 }
 
 // Node continuity (one thread per node): donor-limited link transfers
@@ -321,8 +322,8 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
     double out_sum = 0.0, total_sum = 0.0, unret_sum = 0.0;
     if (n < d.seg_node.mem_end[mem]) {
         const real dt = (real)ctl->dt_real;
-        real vol = d.V[n];
-        if (d.L > 0) {
+        real vol = d.Vp[n];
+        {
             const int k0 = d.nl_ptr[n], k1 = d.nl_ptr[n + 1];
             real delta = zero;
             for (int pass = 0; pass < 2; ++pass) {

@@ -894,6 +894,7 @@ public:
         d_inv.set(inv, s); d_As.set(As, s); d_dmax.set(dmax, s); d_oa.set(oa, s); d_oc.set(oc, s); d_otw.set(otw, s);
         d_ocap.set(ocap, s); d_V.set(V, s); d_nsc.set(nsc, s); d_expl.set(expl, s);
         std::vector<real> zN(NT, 0);
+        d_Vp.set(zN, s);
         d_sfac.set(zN, s); d_over.set(zN, s); d_fb.set(zN, s); d_wsum.set(zN, s);
         d_ni_ptr.set(ni_ptr, s); d_ni_idx.set(ni_idx, s); d_nl_ptr.set(nl_ptr, s); d_nl_ent.set(nl_ent, s);
         d_ne_ptr.set(ne_ptr, s); d_ne_idx.set(ne_idx, s);
@@ -928,7 +929,8 @@ public:
         d.seg_node = seg_nodes.seg(); d.seg_link = seg_links.seg(); d.seg_icell = seg_icl.seg();
         d.seg_ret = seg_rcl.seg(); d.seg_ext = seg_xcl.seg();
         d.inv = d_inv.p; d.As = d_As.p; d.dmax = d_dmax.p; d.o_area = d_oa.p; d.o_coef = d_oc.p; d.o_tw = d_otw.p;
-        d.o_cap = d_ocap.p; d.nsc = d_nsc.p; d.V = d_V.p; d.sur_factor = d_sfac.p; d.overflow = d_over.p;
+        d.o_cap = d_ocap.p; d.nsc = d_nsc.p; d.V = d_V.p; d.Vp = d_Vp.p; d.sur_factor = d_sfac.p;
+        d.overflow = d_over.p;
         d.fallback = d_fb.p; d.wsum = d_wsum.p; d.explicit_ = d_expl.p;
         d.ni_ptr = d_ni_ptr.p; d.ni_idx = d_ni_idx.p; d.nl_ptr = d_nl_ptr.p; d.nl_ent = d_nl_ent.p;
         d.ne_ptr = d_ne_ptr.p; d.ne_idx = d_ne_idx.p;
@@ -996,18 +998,10 @@ public:
             timed("inlet_cell", (double)d.Ci * (12 + 4 * rb) + (double)d.I * (8 + 5 * rb), [&] {
                 fh::k_inlet_cell<real><<<icl.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p, d_area.p, d_part_inlet.p);
             });
-            timed("inlet_node_apply", (double)d.N * (8 + 2 * rb) + (double)d.I * (4 + 2 * rb), [&] {
-                fh::k_inlet_node_apply<real><<<nodes.blocks, 256, 0, s>>>(ctl, d);
-            });
         }
-        if (links.blocks > 0) {
-            timed("link", (double)d.L * (12 + 11 * rb) + (double)d.N * 3 * rb, [&] {
-                fh::k_link<real><<<links.blocks, 256, 0, s>>>(ctl, kp, d);
-            });
-            timed("node_factor", (double)d.N * (4 + 2 * rb) + (double)d.L * (8 + rb), [&] {
-                fh::k_node_factor<real><<<nodes.blocks, 256, 0, s>>>(ctl, d);
-            });
-        }
+        timed("node_links", (double)d.N * (12 + 5 * rb) + (double)d.L * (16 + 12 * rb) + (double)d.I * (4 + 2 * rb), [&] {
+            fh::k_node_links<real><<<nodes.blocks, 256, 0, s>>>(ctl, kp, d);
+        });
         timed("node_update", (double)d.N * (16 + 14 * rb) + (double)d.L * (16 + 3 * rb), [&] {
             fh::k_node_update<real><<<nodes.blocks, 256, 0, s>>>(ctl, kp, d, d_part_node.p);
         });
@@ -1223,7 +1217,7 @@ public:
         d_part_ext;
     DevArray<float> d_rain_s, d_rain_e, d_rain_r, d_ext_t, d_ext_q, d_ext_ov;
     // drainage
-    DevArray<real> d_inv, d_As, d_dmax, d_oa, d_oc, d_otw, d_ocap, d_V, d_sfac, d_over, d_fb, d_wsum;
+    DevArray<real> d_inv, d_As, d_dmax, d_oa, d_oc, d_otw, d_ocap, d_V, d_Vp, d_sfac, d_over, d_fb, d_wsum;
     DevArray<int> d_nsc, d_expl, d_ni_ptr, d_ni_idx, d_nl_ptr, d_nl_ent, d_ne_ptr, d_ne_idx;
     DevArray<int> d_icell, d_inode, d_ic_cells, d_ic_ptr, d_ic_idx;
     DevArray<real> d_capq, d_surq, d_sur, d_cap;

@@ -16,6 +16,10 @@ derive from ``BASE_SEED + workload index``.
 * ``make_dam_break``        -- 10 km x 200 m channel, 200,000 triangles, flat frictionless bed,
                                10 m | 0.001 m dam at x = 5 km, reflective walls.
 * ``make_catchment``        -- one member of the uneven ensemble (log-uniform 20k-300k triangles).
+* ``make_shared_mesh_ensemble`` -- members on one mesh and network with their own rain total
+                               (evenly spaced 40-200 mm over 1 h) and Horton f0/fc/k factors 1+U(-0.2,0.2).
+* ``make_uneven_ensemble``  -- 48 catchments with log-uniform 20k-300k triangles, own hyetographs
+                               (20-200 mm over 1 h) and networks at ~12 % of their triangle count.
 """
 
 from __future__ import annotations
@@ -351,3 +355,35 @@ def make_dam_break(nx: int = 2500, ny: int = 40, length: float = 10000.0, width:
     host = {"nodes": nodes, "triangles": tri, "centroids": cen}
     return _assemble(surface, empty_coupling(), empty_pipe(), empty_recharge(), forcing, h0=h0,
                      host=host, report={"workload": "dam_break", "seed": seed, "triangles": C})
+
+
+def make_shared_mesh_ensemble(base, n_members: int = 16, seed: int = BASE_SEED + 5,
+                              rain_range_mm=(40.0, 200.0)):
+    """Members sharing ``base``'s mesh and network (arrays are shared, not copied)."""
+    rng = np.random.default_rng(seed)
+    totals = np.linspace(rain_range_mm[0], rain_range_mm[1], n_members)
+    out = []
+    f = base.forcing
+    for m in range(n_members):
+        fac = 1.0 + rng.uniform(-0.2, 0.2, 3)
+        forcing = f._replace(
+            rainfall_start_s=np.zeros(1, np.float32), rainfall_end_s=np.full(1, 3600.0, np.float32),
+            rainfall_rate_mps=np.array([totals[m] * MM_HR_TO_M_S], np.float32),
+            horton_f0_mps=(f.horton_f0_mps * np.float32(fac[0])).astype(np.float32),
+            horton_fc_mps=(f.horton_fc_mps * np.float32(fac[1])).astype(np.float32),
+            horton_k_per_s=(f.horton_k_per_s * np.float32(fac[2])).astype(np.float32))
+        out.append(base._replace(forcing=forcing,
+                                 report={**base.report, "member": m, "rain_total_mm": float(totals[m]),
+                                         "horton_factors": fac.tolist()}))
+    return out
+
+
+def make_uneven_ensemble(n_members: int = 48, seed: int = BASE_SEED + 4, lo: float = 2.0e4, hi: float = 3.0e5,
+                         spacing: float = 25.0):
+    """Catchments with log-uniform triangle counts, each with its own storm and network."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), n_members))).astype(int)
+    totals = rng.uniform(20.0, 200.0, n_members)
+    return [make_catchment(int(n), seed=seed * 1000 + i, spacing=spacing, rain_mm_h=[float(totals[i])],
+                           rain_hours=1, interval_s=3600.0)
+            for i, n in enumerate(sizes)]

@@ -344,8 +344,6 @@ def inlet_exchange(state, surface, coupling, dt, dtype=np.float32,
     F = _f(dtype)
     dt = F(dt)
     cell, node = coupling.inlet_cell, coupling.inlet_node
-    if cell.size == 0:
-        return state, 0.0, 0.0
     node_head = _node_head(state.node_volume, coupling, dtype)
     surface_head = surface.bed[cell] + state.h[cell]
     delta_head = surface_head - node_head[node]
@@ -437,8 +435,9 @@ def pipe_links_and_outfalls(state, coupling, pipe, dt, dtype=np.float32, slot_wa
         transfer = np.where(positive, F(1), F(-1)) * requested_abs * factor[donor]
         node_delta = scatter_add(np.concatenate((source, target)),
                                  np.concatenate((-transfer, transfer)), n_nodes)
-        node_volume = np.maximum(node_volume + node_delta, F(0))
+        node_volume = node_volume + node_delta
         link_flow = transfer / np.maximum(dt, F(1.0e-12))
+    node_volume = np.maximum(node_volume, F(0))     # applied to every node, links or not
     head = _node_head(node_volume, coupling, dtype)
     dh = head - pipe.outfall_tailwater
     outfall_q = pipe.outfall_coefficient * pipe.outfall_area * np.sqrt(F(2.0 * GRAVITY) * np.abs(dh)) * np.sign(dh)
