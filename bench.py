@@ -168,9 +168,9 @@ def main():
     s = HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
                         compensated=args.compensated, use_graph=not args.no_graph,
                         block_size=args.block_size, xcd_remap=not args.no_remap)
-    log(f"uploaded: {s.device_bytes() / 2**20:.1f} MiB resident")
     final_t = 3600.0
     s.advance_to(final_t, final_t, max_steps=max(args.warmup, 1), raise_on_budget=False)
+    log(f"resident on the GPU: {s.device_bytes() / 2**20:.1f} MiB")
     log(f"warmup {args.warmup} steps done, t={s.clock()['time_s']:.2f}s")
     c0 = s.clock()
     t0 = time.perf_counter()

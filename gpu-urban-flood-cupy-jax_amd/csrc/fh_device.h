@@ -151,16 +151,21 @@ __device__ __forceinline__ T block_min(T v, T* lds) {
 // Last-block-done: every block of the reduction publishes its partials, then
 // the block that arrives last (grid-wide counter) sums all partials in a fixed
 // order and commits them.  Returns true in the last block (all threads).
+//
+// The partials are written and read with agent-scope atomic accesses, which are
+// coherent across the XCDs' L2 caches, so no cache-wide fence is needed: thread
+// 0 only waits until its own partial stores have completed before it bumps the
+// counter.  (An agent-scope __threadfence here would write back the whole L2 of
+// the XCD once per block.)
 __device__ __forceinline__ bool arrive_last(unsigned* counter, unsigned nblocks) {
     __shared__ int s_last;
-    __threadfence();
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned prev = atomicAdd(counter, 1u);
+        __builtin_amdgcn_s_waitcnt(0);
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_last = (prev == nblocks - 1);
     }
     __syncthreads();
-    if (s_last) __threadfence();
     return s_last != 0;
 }
 

@@ -76,7 +76,7 @@ __device__ __forceinline__ void commit_terms(Ctl* ctl, int counter, const double
     double s[NT];
     sum_partials<NT>(part, nparts, s, lds);
     if (threadIdx.x == 0) {
-        ctl->counters[counter] = 0u;
+        __hip_atomic_store(&ctl->counters[counter], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
             ctl->diag[terms[k]] = s[k];
@@ -276,30 +276,38 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const DrainDev<r
     return cand * dt;
 }
 
-// Links and donor factors (one thread per node): the node's post-inlet volume,
-// the requested transfer of every incident link (recomputed by both end nodes
-// from identical inputs; the 'from' node stores it), and the donor factor
-// min(1, V / sum of |requested| over the links this node donates to, in
-// ascending link order) -- v1_parity.py:348-352.
+// Pipe links (one thread per link): the requested transfer Q dt from the
+// post-inlet volumes of the link's two end nodes (v1_parity.py:324-347).
 template <typename real>
-__global__ void __launch_bounds__(256) k_node_links(const Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d) {
-    const int mem = d.seg_node.blk_member[blockIdx.x];
+__global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d) {
+    const int mem = d.seg_link.blk_member[blockIdx.x];
     const Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= d.seg_link.mem_end[mem]) return;
+    const real dt = (real)ctl->dt_real;
+    const int a = d.from[l], b = d.to[l];
+    const real va = post_inlet_volume(d, a);
+    const real vb = a == b ? va : post_inlet_volume(d, b);
+    d.req[l] = link_request(p, d, l, va, vb, dt);
+}
+
+// Donor factors (one thread per node): the node's post-inlet volume and
+// min(1, V / sum of |requested| over the links it donates to, in ascending link
+// order) -- v1_parity.py:348-352.
+template <typename real>
+__global__ void __launch_bounds__(256) k_node_factor(const Ctl* __restrict__ ctls, DrainDev<real> d) {
+    const int mem = d.seg_node.blk_member[blockIdx.x];
+    if (!ctls[mem].active) return;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= d.seg_node.mem_end[mem]) return;
-    const real dt = (real)ctl->dt_real;
     const real vn = post_inlet_volume(d, n);
     d.Vp[n] = vn;
     real total = real(0);
     for (int k = d.nl_ptr[n]; k < d.nl_ptr[n + 1]; ++k) {
         const int en = d.nl_ent[k];
-        const int l = en >> 1;
+        const real r = d.req[en >> 1];
         const bool is_to = en & 1;
-        const int other = is_to ? d.from[l] : d.to[l];
-        const real vo = other == n ? vn : post_inlet_volume(d, other);
-        const real r = is_to ? link_request(p, d, l, vo, vn, dt) : link_request(p, d, l, vn, vo, dt);
-        if (!is_to) d.req[l] = r;
         const bool positive = r >= real(0);
         if (positive != is_to) total = total + fabs(r);   // donor = from if positive else to
     }

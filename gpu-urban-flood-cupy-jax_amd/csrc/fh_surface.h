@@ -154,7 +154,7 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
     }
     gmin = block_min(gmin, lds);
     if (threadIdx.x != 0) return;
-    ctl->counters[C_BEGIN] = 0u;
+    __hip_atomic_store(&ctl->counters[C_BEGIN], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (eval) ctl->cfl_min = gmin;
     const double t = ctl->t;
     const bool active = (t < ctl->target) && (ctl->interval_step < ctl->max_steps);
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         double s[2];
         sum_partials<2>(part + 2 * (size_t)g0, gn, s, lds);
         if (threadIdx.x == 0) {
-            ctl->counters[C_FLUX] = 0u;
+            __hip_atomic_store(&ctl->counters[C_FLUX], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             ctl->diag[L_BIN] = s[0];
             ctl->diag[L_BOUT] = s[1];
             ctl->diag[L_BNET] = s[0] - s[1];
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
     double sum[2];
     sum_partials<2>(part + 2 * (size_t)blk0, nblk, sum, lds);
     if (threadIdx.x == 0) {
-        ctl->counters[C_UPDATE] = 0u;
+        __hip_atomic_store(&ctl->counters[C_UPDATE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ctl->diag[L_RAIN] = sum[0];
         ctl->diag[L_INFIL] = sum[1];
         ctl->ledger[L_RAIN] += sum[0];
@@ -493,7 +493,7 @@ __global__ void k_finite(Ctl* __restrict__ ctls, int n_cells, const typename V4<
     double s[1];
     sum_partials<1>(part, gridDim.x, s, lds);
     if (threadIdx.x == 0) {
-        ctls[0].counters[C_FINITE] = 0u;
+        __hip_atomic_store(&ctls[0].counters[C_FINITE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ctls[0].nonfinite = (int)s[0];
     }
 }

@@ -999,8 +999,12 @@ public:
                 fh::k_inlet_cell<real><<<icl.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p, d_area.p, d_part_inlet.p);
             });
         }
-        timed("node_links", (double)d.N * (12 + 5 * rb) + (double)d.L * (16 + 12 * rb) + (double)d.I * (4 + 2 * rb), [&] {
-            fh::k_node_links<real><<<nodes.blocks, 256, 0, s>>>(ctl, kp, d);
+        if (links.blocks > 0)
+            timed("link", (double)d.L * (12 + 12 * rb) + (double)d.N * 4 * rb, [&] {
+                fh::k_link<real><<<links.blocks, 256, 0, s>>>(ctl, kp, d);
+            });
+        timed("node_factor", (double)d.N * (8 + 3 * rb) + (double)d.L * (8 + rb) + (double)d.I * (4 + 2 * rb), [&] {
+            fh::k_node_factor<real><<<nodes.blocks, 256, 0, s>>>(ctl, d);
         });
         timed("node_update", (double)d.N * (16 + 14 * rb) + (double)d.L * (16 + 3 * rb), [&] {
             fh::k_node_update<real><<<nodes.blocks, 256, 0, s>>>(ctl, kp, d, d_part_node.p);
