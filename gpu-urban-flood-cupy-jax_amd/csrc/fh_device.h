@@ -58,6 +58,7 @@ struct Ctl {
     int32_t nonfinite;     // count of non-finite values found by k_finite
     int32_t pad0;
     double cfl_min;        // result of the last CFL reduction
+    unsigned long long cfl_bits;  // running min of the CFL reduction (bits of a non-negative double)
     double diag[kLedgerTerms];    // this step's ledger increments
     double ledger[kLedgerTerms];  // cumulative FP64 ledger
     unsigned counters[C_COUNT];
@@ -182,6 +183,17 @@ __device__ __forceinline__ void sum_partials(const double* part, int nparts, dou
             out[k] += __hip_atomic_load(part + (size_t)i * N + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     block_sum<N>(out, lds);
+}
+
+// Adds one block's partials to the block's own accumulation slot.  Only this
+// block ever writes the slot and launches are stream-ordered, so the slots
+// need no atomics; k_fold sums them into the FP64 ledger in a fixed order.
+template <int N>
+__device__ __forceinline__ void accumulate_slot(double* acc, int slot, const double (&v)[N]) {
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) acc[(size_t)slot * N + k] += v[k];
+    }
 }
 
 // Publish one block's partial record (agent-scope stores: visible to the last

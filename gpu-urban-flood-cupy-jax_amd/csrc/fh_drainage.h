@@ -122,11 +122,7 @@ __global__ void __launch_bounds__(256) k_external(Ctl* __restrict__ ctls, KP<rea
     __shared__ double lds[16];
     double v[1] = {added_sum};
     block_sum<1>(v, lds);
-    publish_partials<1>(part, b, v);
-    const int blk0 = d.seg_ext.mem_blk0[mem], nblk = d.seg_ext.mem_nblk[mem];
-    if (!arrive_last(&ctl->counters[C_EXTERNAL], nblk)) return;
-    const int terms[1] = {L_EXTERNAL};
-    commit_terms<1>(ctl, C_EXTERNAL, part + (size_t)blk0, nblk, terms, lds);
+    accumulate_slot<1>(part, b, v);
 }
 
 // Inlet exchange, pass A (one thread per node): surcharge requests of the node's
@@ -206,11 +202,7 @@ __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<r
     __shared__ double lds[32];
     double v[2] = {cap_sum, sur_sum};
     block_sum<2>(v, lds);
-    publish_partials<2>(part, b, v);
-    const int blk0 = d.seg_icell.mem_blk0[mem], nblk = d.seg_icell.mem_nblk[mem];
-    if (!arrive_last(&ctl->counters[C_INLET], nblk)) return;
-    const int terms[2] = {L_CAPTURE, L_SURCHARGE};
-    commit_terms<2>(ctl, C_INLET, part + 2 * (size_t)blk0, nblk, terms, lds);
+    accumulate_slot<2>(part, b, v);
 }
 
 // Node volume after the inlet exchange (v1_parity.py:292-298):
@@ -391,11 +383,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
     __shared__ double lds[48];
     double v[3] = {out_sum, total_sum, unret_sum};
     block_sum<3>(v, lds);
-    publish_partials<3>(part, b, v);
-    const int blk0 = d.seg_node.mem_blk0[mem], nblk = d.seg_node.mem_nblk[mem];
-    if (!arrive_last(&ctl->counters[C_NODE], nblk)) return;
-    const int terms[3] = {L_OUTFALL, L_OVERFLOW, L_UNRETURNED};
-    commit_terms<3>(ctl, C_NODE, part + 3 * (size_t)blk0, nblk, terms, lds);
+    accumulate_slot<3>(part, b, v);
 }
 
 // Return pass (one thread per receiving cell): weighted entry returns, node
@@ -456,11 +444,7 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
     __shared__ double lds[32];
     double v[2] = {cap_sum, loss_sum};
     block_sum<2>(v, lds);
-    publish_partials<2>(part, b, v);
-    const int blk0 = d.seg_ret.mem_blk0[mem], nblk = d.seg_ret.mem_nblk[mem];
-    if (!arrive_last(&ctl->counters[C_RETURN], nblk)) return;
-    const int terms[2] = {L_RCAPTURE, L_RLOSS};
-    commit_terms<2>(ctl, C_RETURN, part + 2 * (size_t)blk0, nblk, terms, lds);
+    accumulate_slot<2>(part, b, v);
 }
 
 }  // namespace fh

@@ -144,18 +144,22 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
         vmin_ = (double)mn;
     }
     vmin_ = block_min(vmin_, lds);
-    double rec[1] = {vmin_};
-    publish_partials<1>(part, b, rec);
-    if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
-    double gmin = INFINITY;
-    for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
-        double v = __hip_atomic_load(part + blk0 + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        gmin = v < gmin ? v : gmin;
+    if (eval && threadIdx.x == 0) {
+        // non-negative doubles order like their bit patterns: one atomic min per block
+        __hip_atomic_fetch_min(&ctl->cfl_bits, (unsigned long long)__double_as_longlong(vmin_), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
-    gmin = block_min(gmin, lds);
+    if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
     if (threadIdx.x != 0) return;
     __hip_atomic_store(&ctl->counters[C_BEGIN], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (eval) ctl->cfl_min = gmin;
+    double gmin = INFINITY;
+    if (eval) {
+        const unsigned long long bits = __hip_atomic_load(&ctl->cfl_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        gmin = __longlong_as_double((long long)bits);
+        __hip_atomic_store(&ctl->cfl_bits, (unsigned long long)__double_as_longlong(INFINITY), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        ctl->cfl_min = gmin;
+    }
     const double t = ctl->t;
     const bool active = (t < ctl->target) && (ctl->interval_step < ctl->max_steps);
     ctl->active = active ? 1 : 0;
@@ -208,7 +212,6 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
         ctl->ext_upper = us;
         ctl->ext_frac = frac;
     }
-    for (int k = 0; k < kLedgerTerms; ++k) ctl->diag[k] = 0.0;
     ctl->t_step = t;
     ctl->t = t + (double)dtr;
     ctl->step += 1;
@@ -278,19 +281,7 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         __shared__ double lds[16 * 2];
         double v[2] = {bin, bout};
         block_sum<2>(v, lds);
-        publish_partials<2>(part, b, v);
-        if (!arrive_last(&ctl->counters[C_FLUX], gn)) return;
-        double s[2];
-        sum_partials<2>(part + 2 * (size_t)g0, gn, s, lds);
-        if (threadIdx.x == 0) {
-            __hip_atomic_store(&ctl->counters[C_FLUX], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ctl->diag[L_BIN] = s[0];
-            ctl->diag[L_BOUT] = s[1];
-            ctl->diag[L_BNET] = s[0] - s[1];
-            ctl->ledger[L_BIN] += s[0];
-            ctl->ledger[L_BOUT] += s[1];
-            ctl->ledger[L_BNET] += s[0] - s[1];
-        }
+        accumulate_slot<2>(part, b, v);
     }
 }
 
@@ -422,17 +413,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
     __shared__ double lds[16 * 2];
     double v[2] = {rain_v, inf_v};
     block_sum<2>(v, lds);
-    publish_partials<2>(part, b, v);
-    if (!arrive_last(&ctl->counters[C_UPDATE], nblk)) return;
-    double sum[2];
-    sum_partials<2>(part + 2 * (size_t)blk0, nblk, sum, lds);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&ctl->counters[C_UPDATE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ctl->diag[L_RAIN] = sum[0];
-        ctl->diag[L_INFIL] = sum[1];
-        ctl->ledger[L_RAIN] += sum[0];
-        ctl->ledger[L_INFIL] += sum[1];
-    }
+    accumulate_slot<2>(part, b, v);
 }
 
 // Applies the pending wet-time increment of each member's last executed step.
@@ -495,6 +476,51 @@ __global__ void k_finite(Ctl* __restrict__ ctls, int n_cells, const typename V4<
     if (threadIdx.x == 0) {
         __hip_atomic_store(&ctls[0].counters[C_FINITE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         ctls[0].nonfinite = (int)s[0];
+    }
+}
+
+// Ledger accumulation slots of one reducing kernel: the member's block range
+// in that kernel's index space, N terms per slot, and the MassLedger terms.
+struct FoldGroup {
+    double* acc;
+    const int* blk0;    // [M] first block of the member (or first ghost block)
+    const int* nblk;    // [M] number of blocks
+    int n;              // terms per slot (1..3)
+    int terms[3];
+};
+
+struct FoldTables {
+    FoldGroup g[6];
+    int ngroups;
+};
+
+// k_fold: one block per member sums the member's accumulation slots (fixed
+// order), writes the sums as the member's diagnostics, adds them to its FP64
+// ledger and clears the slots.
+__global__ void __launch_bounds__(256) k_fold(Ctl* __restrict__ ctls, FoldTables f) {
+    __shared__ double lds[16 * 3];
+    const int m = blockIdx.x;
+    Ctl* ctl = ctls + m;
+    double d[kLedgerTerms];
+    for (int k = 0; k < kLedgerTerms; ++k) d[k] = 0.0;
+    for (int gi = 0; gi < f.ngroups; ++gi) {
+        const FoldGroup& g = f.g[gi];
+        const int b0 = g.blk0[m], nb = g.nblk[m];
+        double v[3] = {0.0, 0.0, 0.0};
+        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+            double* slot = g.acc + (size_t)(b0 + i) * g.n;
+            for (int k = 0; k < g.n; ++k) { v[k] += slot[k]; slot[k] = 0.0; }
+        }
+        block_sum<3>(v, lds);
+        if (threadIdx.x == 0)
+            for (int k = 0; k < g.n; ++k) d[g.terms[k]] = v[k];
+    }
+    if (threadIdx.x == 0) {
+        d[L_BNET] = d[L_BIN] - d[L_BOUT];
+        for (int k = 0; k < kLedgerTerms; ++k) {
+            ctl->diag[k] = d[k];
+            ctl->ledger[k] += d[k];
+        }
     }
 }
 

@@ -237,6 +237,8 @@ public:
         for (auto& m : mem) {
             std::memset(&m.ctl, 0, sizeof(fh::Ctl));
             m.ctl.cached_cfl = 1.0;
+            const double inf = INFINITY;
+            std::memcpy(&m.ctl.cfl_bits, &inf, sizeof(inf));
             m.ctl.eval_next = P.fixed_dt > 0.0 ? 0 : 1;
         }
         for (int m = 0; m < M; ++m) ctl_h[m] = mem[m].ctl;
@@ -656,7 +658,7 @@ public:
         // k_begin blocks: proportional to the member's cells
         {
             std::vector<int64_t> nb(M);
-            for (int i = 0; i < M; ++i) nb[i] = std::max<int64_t>(1, std::min<int64_t>(256, (mem[order[i]].C + 2047) / 2048));
+            for (int i = 0; i < M; ++i) nb[i] = std::max<int64_t>(1, std::min<int64_t>(128, (mem[order[i]].C + 8191) / 8192));
             begin_l.base.assign(M, 0); begin_l.count.assign(M, 0);
             begin_l.blk_member.clear(); begin_l.mem_blk0.assign(M, 0); begin_l.mem_nblk.assign(M, 0);
             begin_l.mem_end.assign(M, 0);
@@ -727,8 +729,8 @@ public:
         d_rain_off.set(by_id(rain_off, order), s); d_rain_n.set(by_id(rain_n, order), s);
         d_ext_off.set(by_id(ext_off, order), s); d_ext_n.set(by_id(ext_n, order), s);
         d_part_begin.alloc(std::max(begin_l.blocks, 1));
-        d_part_flux.alloc(2 * (size_t)std::max(edges.blocks, 1));
-        d_part_upd.alloc(2 * (size_t)std::max(cells.blocks, 1));
+        d_part_flux.set(std::vector<double>(2 * (size_t)std::max(edges.blocks, 1), 0.0), s);
+        d_part_upd.set(std::vector<double>(2 * (size_t)std::max(cells.blocks, 1), 0.0), s);
         d_part_fin.alloc(1024);
         pack_drainage(icells, rcells, xcells);
         // control blocks: per member, unchanged
@@ -916,10 +918,10 @@ public:
         d_xc_cells.set(xc_cells, s); d_xc_ptr.set(xc_ptr, s); d_xc_idx.set(xc_idx, s);
         seg_nodes.set(nodes, order, s); seg_links.set(links, order, s); seg_icl.set(icl, order, s);
         seg_rcl.set(rcl, order, s); seg_xcl.set(xcl, order, s);
-        d_part_inlet.alloc(2 * (size_t)std::max(icl.blocks, 1));
-        d_part_node.alloc(3 * (size_t)std::max(nodes.blocks, 1));
-        d_part_ret.alloc(2 * (size_t)std::max(rcl.blocks, 1));
-        d_part_ext.alloc((size_t)std::max(xcl.blocks, 1));
+        d_part_inlet.set(std::vector<double>(2 * (size_t)std::max(icl.blocks, 1), 0.0), s);
+        d_part_node.set(std::vector<double>(3 * (size_t)std::max(nodes.blocks, 1), 0.0), s);
+        d_part_ret.set(std::vector<double>(2 * (size_t)std::max(rcl.blocks, 1), 0.0), s);
+        d_part_ext.set(std::vector<double>((size_t)std::max(xcl.blocks, 1), 0.0), s);
         n_receivers = Xt;
         // device view
         fh::DrainDev<real>& d = dd;
@@ -1048,6 +1050,22 @@ public:
         }
     }
 
+    // Sum every member's ledger accumulation slots into its FP64 ledger.
+    void fold() {
+        fh::FoldTables f{};
+        auto group = [&](int i, double* acc, const DevSeg& seg, int n, int t0, int t1, int t2) {
+            f.g[i] = fh::FoldGroup{acc, seg.mem_blk0.p, seg.mem_nblk.p, n, {t0, t1, t2}};
+        };
+        group(0, d_part_upd.p, seg_cells, 2, fh::L_RAIN, fh::L_INFIL, 0);
+        f.g[1] = fh::FoldGroup{d_part_flux.p, d_gblk0.p, d_gnblk.p, 2, {fh::L_BIN, fh::L_BOUT, 0}};
+        group(2, d_part_inlet.p, seg_icl, 2, fh::L_CAPTURE, fh::L_SURCHARGE, 0);
+        group(3, d_part_node.p, seg_nodes, 3, fh::L_OUTFALL, fh::L_OVERFLOW, fh::L_UNRETURNED);
+        group(4, d_part_ret.p, seg_rcl, 2, fh::L_RCAPTURE, fh::L_RLOSS, 0);
+        group(5, d_part_ext.p, seg_xcl, 1, fh::L_EXTERNAL, 0, 0);
+        f.ngroups = 6;
+        fh::k_fold<<<M, 256, 0, stream>>>(ctl, f);
+    }
+
     void finalize_pending() {
         fh::k_finalize_wet<real><<<std::max(cells.blocks, 1), bs, 0, stream>>>(ctl, kp, seg_cells.seg(), d_state.p,
                                                                                d_wet.p);
@@ -1088,6 +1106,7 @@ public:
             for (int i = 0; i < chunks; ++i) launch_chunk(K);
             pull_ctl();
         }
+        fold();
         finalize_pending();
         const bool finite = check_finite();
         if (!finite) return FH_ERR_SIMULATION;
@@ -1109,10 +1128,12 @@ public:
             d_ext_ov.set(ov, stream);
             dd.use_override = 1;
         }
+        fold();   // commits anything pending, so the next fold holds this step alone
         fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, INFINITY, INFINITY, 1);
         fh::k_set_override<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, 1, dt, rain);
         launch_step();
         fh::k_set_override<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, 0, 0.0, 0.0);
+        fold();
         finalize_pending();
         dd.use_override = 0;
         pull_ctl();
@@ -1183,6 +1204,7 @@ public:
             timed("update", upd_bytes, [&] { launch_update(); });
             launch_drainage(timed);
         }
+        fold();
         finalize_pending();
         FH_CHECK(hipStreamSynchronize(stream));
         return out;
