@@ -35,6 +35,23 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "hbm_traffic_flux_update_981880.md"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy-kernel factor);
+    None when the summary is absent."""
+    try:
+        for line in TRAFFIC_FILE.read_text().splitlines():
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) >= 5 and cells[0] == kernel:
+                return float(cells[4])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def survey_bytes(kernel, E, C, compensated=False):
     """Algorithmic bytes per launch (float32): each required element once per step."""
     if kernel == "flux":
@@ -137,7 +154,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--compensated", action="store_true")
@@ -149,6 +166,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--uneven", action="store_true", help="also run the 48-member uneven ensemble")
+    ap.add_argument("--copy-probe-mib", type=int, default=256,
+                    help="size of the streaming-copy probe (achievable HBM bandwidth); 0 skips it")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -204,6 +223,16 @@ def main():
     if not args.no_batch and args.batch_members > 0:
         batched = batch_leg(inp, args, tri_steps)
     uneven = batch_uneven_leg(args) if args.uneven else None
+    copy = None
+    if args.copy_probe_mib > 0:
+        import ctypes
+        from floodhip import _lib
+        nf = args.copy_probe_mib * 2**20 // 4
+        ms = ctypes.c_double()
+        if _lib.load().fh_copy_probe(nf, 20, ctypes.byref(ms)) == 0:
+            copy = {"bytes_per_launch": 2 * nf * 4, "avg_us": 1e3 * ms.value,
+                    "GBps": 2 * nf * 4 / (ms.value * 1e-3) / 1e9}
+            log(f"copy probe: {copy['GBps']:.1f} GB/s ({copy['bytes_per_launch'] / 2**20:.0f} MiB per launch)")
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU baseline: oracle, {args.cpu_steps} steps of the same workload")
@@ -232,8 +261,10 @@ def main():
                    "seed": int(inp.report.get("seed", 0)), "compensated": bool(args.compensated),
                    "hip_graph": not args.no_graph, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if args.compensated else measured_traffic(dom["kernel"])},
         "kernels": kernels,
+        "copy_probe": copy,
         "batched_shared_mesh": batched,
         "batched_uneven": uneven,
         "cpu_baseline": cpu,

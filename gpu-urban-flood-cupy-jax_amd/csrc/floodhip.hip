@@ -9,6 +9,7 @@
 // kernel launches covering every member; fh_advance_to replays captured
 // hipGraphs of steps_per_launch steps and reads the control blocks back once
 // per replay batch.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -186,7 +187,86 @@ struct Member {
     // state (host copy; authoritative only while the batch is not packed)
     std::vector<real> h, hu, hv, wet, V, Q, rstore;
     fh::Ctl ctl{};
+    // device layout: cpos[c] = position of the caller's cell c, epos[i] = position
+    // of staged edge i (both member-local; empty = as given)
+    std::vector<int> cpos, epos;
+    int cp(int c) const { return cpos.empty() ? c : cpos[(size_t)c]; }
 };
+
+// Breadth-first (Cuthill-McKee) order of the cells over interior edges: each
+// component starts at a pseudo-peripheral cell, neighbours are visited by
+// increasing degree.  Cells that share an edge end up a front width apart, so
+// the gathers of the flux and update stages stay within the L2 working set.
+// Edges follow their lower-numbered cell (interior block and ghost block each
+// keep their place), ties in staged order.
+template <typename real>
+void locality_order(Member<real>& m) {
+    const int64_t C = m.C, Ei = m.E_int, E = m.E;
+    std::vector<int> deg((size_t)C + 1, 0);
+    for (int64_t e = 0; e < Ei; ++e) { deg[(size_t)m.el[e] + 1]++; deg[(size_t)m.er[e] + 1]++; }
+    for (int64_t c = 0; c < C; ++c) deg[(size_t)c + 1] += deg[(size_t)c];
+    std::vector<int> adj((size_t)deg[(size_t)C]);
+    {
+        std::vector<int> fill(deg.begin(), deg.end() - 1);
+        for (int64_t e = 0; e < Ei; ++e) {
+            adj[(size_t)fill[(size_t)m.el[e]]++] = m.er[e];
+            adj[(size_t)fill[(size_t)m.er[e]]++] = m.el[e];
+        }
+    }
+    auto degree = [&](int c) { return deg[(size_t)c + 1] - deg[(size_t)c]; };
+    std::vector<int> order;
+    order.reserve((size_t)C);
+    std::vector<int> mark((size_t)C, 0);   // 0 unseen, else stamp of the BFS that saw it
+    int stamp = 0;
+    std::vector<int> queue;
+    queue.reserve((size_t)C);
+    std::vector<int> nb;
+    // BFS from `root` over unplaced cells; returns the visit list (in queue)
+    auto bfs = [&](int root, bool place) {
+        ++stamp;
+        queue.clear();
+        queue.push_back(root);
+        mark[(size_t)root] = place ? -1 : stamp;
+        for (size_t q = 0; q < queue.size(); ++q) {
+            const int c = queue[q];
+            nb.assign(adj.begin() + deg[(size_t)c], adj.begin() + deg[(size_t)c + 1]);
+            std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return degree(a) < degree(b); });
+            for (int x : nb) {
+                const int mk = mark[(size_t)x];
+                if (mk == -1 || mk == (place ? -1 : stamp)) continue;
+                mark[(size_t)x] = place ? -1 : stamp;
+                queue.push_back(x);
+            }
+        }
+    };
+    for (int64_t c0 = 0; c0 < C; ++c0) {
+        if (mark[(size_t)c0] == -1) continue;
+        int root = (int)c0;
+        // two sweeps towards a peripheral cell: the last cell reached, lowest degree among the last ones
+        for (int sweep = 0; sweep < 2; ++sweep) {
+            bfs(root, false);
+            int best = queue.back();
+            for (size_t q = queue.size(); q-- > 0 && queue.size() - q <= 64;)
+                if (degree(queue[q]) < degree(best)) best = queue[q];
+            root = best;
+        }
+        bfs(root, true);
+        order.insert(order.end(), queue.begin(), queue.end());
+    }
+    m.cpos.assign((size_t)C, 0);
+    for (int64_t i = 0; i < C; ++i) m.cpos[(size_t)order[(size_t)i]] = (int)i;
+    // edges: by the lower new cell index (interior), by the new left cell (ghost)
+    std::vector<int64_t> idx((size_t)E);
+    std::iota(idx.begin(), idx.end(), 0);
+    auto key = [&](int64_t e) {
+        const int l = m.cpos[(size_t)m.el[e]];
+        return e < Ei ? std::min(l, m.cpos[(size_t)m.er[e]]) : l;
+    };
+    std::stable_sort(idx.begin(), idx.begin() + Ei, [&](int64_t a, int64_t b) { return key(a) < key(b); });
+    std::stable_sort(idx.begin() + Ei, idx.end(), [&](int64_t a, int64_t b) { return key(a) < key(b); });
+    m.epos.assign((size_t)E, 0);
+    for (int64_t i = 0; i < E; ++i) m.epos[(size_t)idx[(size_t)i]] = (int)i;
+}
 
 class EngineBase {
 public:
@@ -316,6 +396,8 @@ public:
             ghost_kind.push_back(kind);
         }
         s.C = C_;
+        s.cpos.clear();
+        s.epos.clear();
         s.E_int = (int64_t)interior.size();
         s.E = s.E_int + (int64_t)ghost.size();
         std::vector<int32_t> pos_of((size_t)E_, -1);
@@ -481,12 +563,17 @@ public:
             d_state.get(st.data(), cb, s.C, stream);
             FH_CHECK(hipStreamSynchronize(stream));
             for (int64_t c = 0; c < s.C; ++c) {
-                if (h) st[c].x = (real)h[c];
-                if (hu) st[c].y = (real)hu[c];
-                if (hv) st[c].z = (real)hv[c];
+                const int q = s.cp((int)c);
+                if (h) st[q].x = (real)h[c];
+                if (hu) st[q].y = (real)hu[c];
+                if (hv) st[q].z = (real)hv[c];
             }
             d_state.put(st.data(), cb, s.C, stream);
-            if (wet) { auto w = conv<real>(wet, s.C); d_wet.put(w.data(), cb, s.C, stream); }
+            if (wet) {
+                std::vector<real> w((size_t)s.C);
+                for (int64_t c = 0; c < s.C; ++c) w[(size_t)s.cp((int)c)] = (real)wet[c];
+                d_wet.put(w.data(), cb, s.C, stream);
+            }
             if (P.compensated && (h || hu || hv)) {
                 std::vector<V> z((size_t)s.C);
                 for (auto& v : z) { v.x = 0; v.y = 0; v.z = 0; v.w = 0; }
@@ -522,10 +609,11 @@ public:
         if (s.R) d_rstore.get(r.data(), rrow_base[mp], s.R, stream);
         FH_CHECK(hipStreamSynchronize(stream));
         for (int64_t c = 0; c < s.C; ++c) {
-            if (h) h[c] = (float)st[c].x;
-            if (hu) hu[c] = (float)st[c].y;
-            if (hv) hv[c] = (float)st[c].z;
-            if (wet) wet[c] = (float)w[c];
+            const int q = s.cp((int)c);
+            if (h) h[c] = (float)st[q].x;
+            if (hu) hu[c] = (float)st[q].y;
+            if (hv) hv[c] = (float)st[q].z;
+            if (wet) wet[c] = (float)w[q];
         }
         if (vol) for (int64_t n = 0; n < s.N; ++n) vol[n] = (float)v[n];
         if (q) for (int64_t l = 0; l < s.L; ++l) q[l] = (float)qq[l];
@@ -539,9 +627,10 @@ public:
         d_state.get(st.data(), cells.base[pos_of_member(cur)], s.C, stream);
         FH_CHECK(hipStreamSynchronize(stream));
         for (int64_t c = 0; c < s.C; ++c) {
-            if (h) h[c] = (double)st[c].x;
-            if (hu) hu[c] = (double)st[c].y;
-            if (hv) hv[c] = (double)st[c].z;
+            const int q = s.cp((int)c);
+            if (h) h[c] = (double)st[q].x;
+            if (hu) hu[c] = (double)st[q].y;
+            if (hv) hv[c] = (double)st[q].z;
         }
     }
 
@@ -594,13 +683,17 @@ public:
             if (!s.has_surface) continue;
             const int mp = pos_of_member(m);
             std::vector<V> st((size_t)s.C);
+            std::vector<real> w((size_t)s.C);
             d_state.get(st.data(), cells.base[mp], s.C, stream);
-            d_wet.get(s.wet.data(), cells.base[mp], s.C, stream);
+            d_wet.get(w.data(), cells.base[mp], s.C, stream);
             if (s.N && (int64_t)s.V.size() == s.N) d_V.get(s.V.data(), nodes.base[mp], s.N, stream);
             if (s.L && (int64_t)s.Q.size() == s.L) d_Q.get(s.Q.data(), links.base[mp], s.L, stream);
             if (s.R && (int64_t)s.rstore.size() == s.R) d_rstore.get(s.rstore.data(), rrow_base[mp], s.R, stream);
             FH_CHECK(hipStreamSynchronize(stream));
-            for (int64_t c = 0; c < s.C; ++c) { s.h[c] = st[c].x; s.hu[c] = st[c].y; s.hv[c] = st[c].z; }
+            for (int64_t c = 0; c < s.C; ++c) {
+                const int q = s.cp((int)c);
+                s.h[c] = st[q].x; s.hu[c] = st[q].y; s.hv[c] = st[q].z; s.wet[c] = w[q];
+            }
         }
     }
 
@@ -612,6 +705,10 @@ public:
 
     void pack() {
         drop_graph();
+        for (auto& m : mem) {
+            if (P.cell_order == 1 && m.cpos.empty() && m.C > 0) locality_order(m);
+            if (P.cell_order != 1) { m.cpos.clear(); m.epos.clear(); }
+        }
         // member order in the concatenation
         order.resize(M);
         std::iota(order.begin(), order.end(), 0);
@@ -639,17 +736,17 @@ public:
         for (int i = 0; i < M; ++i) {
             const Member<real>& s = mem[order[i]];
             std::vector<char> mark((size_t)s.C, 0);
-            for (int64_t k = 0; k < s.I; ++k) mark[s.icell[k]] = 1;
+            for (int64_t k = 0; k < s.I; ++k) mark[s.cp(s.icell[k])] = 1;
             for (int64_t c = 0; c < s.C; ++c) if (mark[c]) icells[i].push_back((int)c);
             std::fill(mark.begin(), mark.end(), 0);
-            for (int64_t k = 0; k < s.S; ++k) if (s.snode[k] >= 0 && s.scell[k] >= 0) mark[s.scell[k]] = 1;
+            for (int64_t k = 0; k < s.S; ++k) if (s.snode[k] >= 0 && s.scell[k] >= 0) mark[s.cp(s.scell[k])] = 1;
             std::vector<int> expl((size_t)s.N, 0);
             for (int64_t k = 0; k < s.S; ++k) if (s.snode[k] >= 0 && s.scell[k] >= 0) expl[s.snode[k]] = 1;
-            for (int64_t n = 0; n < s.N; ++n) if (!expl[n] && s.nsc[n] >= 0) mark[s.nsc[n]] = 1;
-            for (int64_t r = 0; r < s.R; ++r) mark[s.rcell[r]] = 1;
+            for (int64_t n = 0; n < s.N; ++n) if (!expl[n] && s.nsc[n] >= 0) mark[s.cp(s.nsc[n])] = 1;
+            for (int64_t r = 0; r < s.R; ++r) mark[s.cp(s.rcell[r])] = 1;
             for (int64_t c = 0; c < s.C; ++c) if (mark[c]) rcells[i].push_back((int)c);
             std::fill(mark.begin(), mark.end(), 0);
-            for (int64_t j = 0; j < s.Rcv; ++j) mark[s.rcv[j]] = 1;
+            for (int64_t j = 0; j < s.Rcv; ++j) mark[s.cp(s.rcv[j])] = 1;
             for (int64_t c = 0; c < s.C; ++c) if (mark[c]) xcells[i].push_back((int)c);
         }
         icl.build(counts_of(icells), 256);
@@ -687,22 +784,29 @@ public:
         for (int i = 0; i < M; ++i) {
             const Member<real>& m = mem[order[i]];
             const int cb = cells.base[i], eb = edges.base[i];
+            const bool perm = !m.epos.empty();
             for (int64_t e = 0; e < m.E; ++e) {
-                el[eb + e] = m.el[e] + cb;
-                er[eb + e] = m.er[e] >= 0 ? m.er[e] + cb : m.er[e];
-                nx[eb + e] = m.nx[e]; ny[eb + e] = m.ny[e]; len[eb + e] = m.len[e];
+                const int64_t q = eb + (perm ? m.epos[(size_t)e] : e);
+                el[q] = m.cp(m.el[e]) + cb;
+                er[q] = m.er[e] >= 0 ? m.cp(m.er[e]) + cb : m.er[e];
+                nx[q] = m.nx[e]; ny[q] = m.ny[e]; len[q] = m.len[e];
             }
             const int tile = edges.tile;
             if (m.E > m.E_int) {
                 gblk0[i] = (eb + (int)m.E_int) / tile;
                 gnblk[i] = (eb + (int)m.E - 1) / tile + 1 - gblk0[i];
             }
-            auto enc = [&](int x) { return x < 0 ? x : ((((x >> 1) + eb) << 1) | (x & 1)); };
+            auto enc = [&](int x) {
+                if (x < 0) return x;
+                const int p = perm ? m.epos[(size_t)(x >> 1)] : (x >> 1);
+                return ((p + eb) << 1) | (x & 1);
+            };
             for (int64_t c = 0; c < m.C; ++c) {
-                ce0[cb + c] = enc(m.ce0[c]); ce1[cb + c] = enc(m.ce1[c]); ce2[cb + c] = enc(m.ce2[c]);
-                st[cb + c].x = m.h[c]; st[cb + c].y = m.hu[c]; st[cb + c].z = m.hv[c]; st[cb + c].w = m.bed[c];
-                wet[cb + c] = m.wet[c]; area[cb + c] = m.area[c]; n2[cb + c] = m.n2[c]; lc[cb + c] = m.lc[c];
-                rainm[cb + c] = m.rainm[c]; f0[cb + c] = m.f0[c]; fc[cb + c] = m.fc[c]; hk[cb + c] = m.hk[c];
+                const int64_t q = cb + m.cp((int)c);
+                ce0[q] = enc(m.ce0[c]); ce1[q] = enc(m.ce1[c]); ce2[q] = enc(m.ce2[c]);
+                st[q].x = m.h[c]; st[q].y = m.hu[c]; st[q].z = m.hv[c]; st[q].w = m.bed[c];
+                wet[q] = m.wet[c]; area[q] = m.area[c]; n2[q] = m.n2[c]; lc[q] = m.lc[c];
+                rainm[q] = m.rainm[c]; f0[q] = m.f0[c]; fc[q] = m.fc[c]; hk[q] = m.hk[c];
             }
             cell0[i] = cb; cell1[i] = cb + (int)m.C;
             rain_off[i] = (int)rs_all.size(); rain_n[i] = (int)m.rain_s.size();
@@ -789,10 +893,10 @@ public:
                 inv[nb + n] = m.inv[n]; As[nb + n] = m.As[n]; dmax[nb + n] = m.dmax[n];
                 oa[nb + n] = m.oarea[n]; oc[nb + n] = m.ocoef[n]; otw[nb + n] = m.otw[n]; ocap[nb + n] = m.ocap[n];
                 V[nb + n] = m.V[n];
-                nsc[nb + n] = m.nsc[n] >= 0 ? m.nsc[n] + cb : -1;
+                nsc[nb + n] = m.nsc[n] >= 0 ? m.cp(m.nsc[n]) + cb : -1;
             }
             for (int64_t k = 0; k < m.I; ++k) {
-                icell[ibase[i] + k] = m.icell[k] + cb;
+                icell[ibase[i] + k] = m.cp(m.icell[k]) + cb;
                 inode[ibase[i] + k] = m.inode[k] + nb;
                 capq[ibase[i] + k] = m.capq[k]; surq[ibase[i] + k] = m.surq[k];
             }
@@ -852,7 +956,7 @@ public:
             }
             std::vector<int> rows(It, -1);
             (void)rows;
-            for (int64_t k = 0; k < m.I; ++k) ic_ptr[slot[m.icell[k]] + 1]++;
+            for (int64_t k = 0; k < m.I; ++k) ic_ptr[slot[m.cp(m.icell[k])] + 1]++;
         }
         for (int j = 0; j < icl.total; ++j) ic_ptr[j + 1] += ic_ptr[j];
         ic_idx.assign(ic_ptr[icl.total], 0);
@@ -862,7 +966,7 @@ public:
                 const Member<real>& m = mem[order[i]];
                 std::vector<int> slot((size_t)m.C, -1);
                 for (size_t j = 0; j < icells[i].size(); ++j) slot[icells[i][j]] = icl.base[i] + (int)j;
-                for (int64_t k = 0; k < m.I; ++k) ic_idx[fill[slot[m.icell[k]]]++] = ibase[i] + (int)k;
+                for (int64_t k = 0; k < m.I; ++k) ic_idx[fill[slot[m.cp(m.icell[k])]]++] = ibase[i] + (int)k;
             }
         }
         // return cells -> entries, fallback nodes, recharge rows; receiver cells -> receivers
@@ -880,13 +984,13 @@ public:
                     xc_cells[xcl.base[i] + j] = xcells[i][j] + cells.base[i];
                 }
                 for (int64_t k = 0; k < m.S; ++k)
-                    if (m.snode[k] >= 0 && m.scell[k] >= 0) erow[sbase[i] + k] = slot[m.scell[k]];
+                    if (m.snode[k] >= 0 && m.scell[k] >= 0) erow[sbase[i] + k] = slot[m.cp(m.scell[k])];
                 for (int64_t n = 0; n < m.N; ++n) {
                     const int g = nodes.base[i] + (int)n;
-                    if (!expl[g] && m.nsc[n] >= 0) frow[g] = slot[m.nsc[n]];
+                    if (!expl[g] && m.nsc[n] >= 0) frow[g] = slot[m.cp(m.nsc[n])];
                 }
-                for (int64_t r = 0; r < m.R; ++r) rrow[rbase[i] + r] = slot[m.rcell[r]];
-                for (int64_t j = 0; j < m.Rcv; ++j) xrow[xbase[i] + j] = xslot[m.rcv[j]];
+                for (int64_t r = 0; r < m.R; ++r) rrow[rbase[i] + r] = slot[m.cp(m.rcell[r])];
+                for (int64_t j = 0; j < m.Rcv; ++j) xrow[xbase[i] + j] = xslot[m.cp(m.rcv[j])];
             }
             build_csr(erow, rcl.total, rc_eptr, rc_eidx);
             build_csr(frow, rcl.total, rc_fptr, rc_fidx);
@@ -954,17 +1058,29 @@ public:
         return fh::BeginTables{seg_begin.seg(), d_cell0.p, d_cell1.p, d_rain_off.p, d_rain_n.p, d_ext_off.p, d_ext_n.p};
     }
 
+    // Stage launches go through here.  While a kernel is being timed the launch
+    // carries start/stop events in its dispatch packet (hipExtLaunchKernel), so
+    // the measured interval is the kernel's own execution, as a profiler sees it.
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    template <typename... KA, typename... A>
+    void go(void (*kernel)(KA...), unsigned grid, unsigned block, A... args) {
+        if (ev_start)
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, ev_start, ev_stop, 0u,
+                                  static_cast<KA>(args)...);
+        else
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, static_cast<KA>(args)...);
+    }
+
     void launch_begin() {
-        fh::k_begin<real><<<begin_l.blocks, 256, 0, stream>>>(ctl, kp, d_state.p, d_lc.p, begin_tables(),
-                                                              d_part_begin.p, d_rain_s.p, d_rain_e.p, d_rain_r.p,
-                                                              d_ext_t.p);
+        go(fh::k_begin<real>, begin_l.blocks, 256, ctl, kp, d_state.p, d_lc.p, begin_tables(), d_part_begin.p,
+           d_rain_s.p, d_rain_e.p, d_rain_r.p, d_ext_t.p);
     }
     void launch_flux() {
         if (edges.blocks == 0) return;
         fh::EdgeSeg es{seg_edges.seg(), d_gblk0.p, d_gnblk.p};
 #define FH_FLUX(EPT_)                                                                                  \
-        fh::k_flux<real, EPT_><<<edges.blocks, bs, 0, stream>>>(ctl, kp, es, d_el.p, d_er.p, d_nx.p, d_ny.p, \
-                                                                d_len.p, d_state.p, d_rec.p, d_recR.p, d_part_flux.p)
+        go(fh::k_flux<real, EPT_>, edges.blocks, bs, ctl, kp, es, d_el.p, d_er.p, d_nx.p, d_ny.p, d_len.p, \
+           d_state.p, d_rec.p, d_recR.p, d_part_flux.p)
         if (ept == 4) FH_FLUX(4);
         else if (ept == 2) FH_FLUX(2);
         else FH_FLUX(1);
@@ -972,15 +1088,13 @@ public:
     }
     void launch_update() {
         if (P.compensated)
-            fh::k_update<real, true><<<cells.blocks, bs, 0, stream>>>(ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p,
-                                                                     d_ce2.p, d_rec.p, d_recR.p, d_state.p, d_comp.p,
-                                                                     d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p,
-                                                                     d_fc.p, d_k.p, d_part_upd.p);
+            go(fh::k_update<real, true>, cells.blocks, bs, ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p, d_ce2.p, d_rec.p,
+               d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p,
+               d_part_upd.p);
         else
-            fh::k_update<real, false><<<cells.blocks, bs, 0, stream>>>(ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p,
-                                                                      d_ce2.p, d_rec.p, d_recR.p, d_state.p, d_comp.p,
-                                                                      d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p,
-                                                                      d_fc.p, d_k.p, d_part_upd.p);
+            go(fh::k_update<real, false>, cells.blocks, bs, ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p, d_ce2.p, d_rec.p,
+               d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p,
+               d_part_upd.p);
     }
 
     template <class Timed>
@@ -990,30 +1104,30 @@ public:
         const fh::DrainDev<real> d = dd;
         if (xcl.blocks > 0)
             timed("external", (double)d.Rcv * 12.0 + d.Cx * 16.0, [&] {
-                fh::k_external<real><<<xcl.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p, d_area.p, d_part_ext.p);
+                go(fh::k_external<real>, xcl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_ext.p);
             });
         if (nodes.blocks == 0) return;
         if (icl.blocks > 0) {
             timed("inlet_node_factor", (double)d.N * (8 + 3 * rb) + (double)d.I * (12 + 5 * rb), [&] {
-                fh::k_inlet_node_factor<real><<<nodes.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p);
+                go(fh::k_inlet_node_factor<real>, nodes.blocks, 256, ctl, kp, d, d_state.p);
             });
             timed("inlet_cell", (double)d.Ci * (12 + 4 * rb) + (double)d.I * (8 + 5 * rb), [&] {
-                fh::k_inlet_cell<real><<<icl.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p, d_area.p, d_part_inlet.p);
+                go(fh::k_inlet_cell<real>, icl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_inlet.p);
             });
         }
         if (links.blocks > 0)
             timed("link", (double)d.L * (12 + 12 * rb) + (double)d.N * 4 * rb, [&] {
-                fh::k_link<real><<<links.blocks, 256, 0, s>>>(ctl, kp, d);
+                go(fh::k_link<real>, links.blocks, 256, ctl, kp, d);
             });
         timed("node_factor", (double)d.N * (8 + 3 * rb) + (double)d.L * (8 + rb) + (double)d.I * (4 + 2 * rb), [&] {
-            fh::k_node_factor<real><<<nodes.blocks, 256, 0, s>>>(ctl, d);
+            go(fh::k_node_factor<real>, nodes.blocks, 256, ctl, d);
         });
         timed("node_update", (double)d.N * (16 + 14 * rb) + (double)d.L * (16 + 3 * rb), [&] {
-            fh::k_node_update<real><<<nodes.blocks, 256, 0, s>>>(ctl, kp, d, d_part_node.p);
+            go(fh::k_node_update<real>, nodes.blocks, 256, ctl, kp, d, d_part_node.p);
         });
         if (rcl.blocks > 0)
             timed("return", (double)d.Cr * (16 + 3 * rb) + (double)d.N * 2 * rb, [&] {
-                fh::k_return<real><<<rcl.blocks, 256, 0, s>>>(ctl, kp, d, d_state.p, d_area.p, d_part_ret.p);
+                go(fh::k_return<real>, rcl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_ret.p);
             });
     }
 
@@ -1174,9 +1288,11 @@ public:
             hipEvent_t a, b;
             FH_CHECK(hipEventCreate(&a));
             FH_CHECK(hipEventCreate(&b));
-            FH_CHECK(hipEventRecord(a, stream));
+            ev_start = a;
+            ev_stop = b;
             fn();
-            FH_CHECK(hipEventRecord(b, stream));
+            ev_start = ev_stop = nullptr;
+            FH_CHECK(hipGetLastError());
             FH_CHECK(hipEventSynchronize(b));
             float ms = 0.f;
             FH_CHECK(hipEventElapsedTime(&ms, a, b));
@@ -1265,6 +1381,14 @@ struct Ctx {
     std::string err;
 };
 
+// Plain streaming copy: the reference point for HBM byte counters and for the
+// achievable bandwidth (reads and writes n floats once).
+__global__ void __launch_bounds__(256) k_copy_probe(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                    int64_t n4) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
 
 struct fh_ctx {
@@ -1312,6 +1436,7 @@ void fh_default_params(fh_params* p) {
     p->device = 0;
     p->edges_per_thread = 1;
     p->pack_order = 0;
+    p->cell_order = 1;
 }
 
 const char* fh_version(void) { return "floodhip 0.2.0 (gfx950)"; }
@@ -1495,5 +1620,36 @@ int fh_profile_kernels(fh_ctx* ctx, int32_t steps, int32_t cap, int32_t* count, 
 }
 
 int64_t fh_device_bytes(const fh_ctx* ctx) { return ctx ? ctx->c.eng->device_bytes() : 0; }
+
+int fh_copy_probe(int64_t n_floats, int32_t reps, double* avg_ms) {
+    if (n_floats <= 0 || (n_floats & 3) || reps <= 0 || !avg_ms) return FH_ERR_VALUE;
+    float* a = nullptr;
+    float* b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int status = FH_OK;
+    if (hipMalloc((void**)&a, n_floats * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&b, n_floats * sizeof(float)) != hipSuccess ||
+        hipMemset(a, 0, n_floats * sizeof(float)) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        status = FH_ERR_RUNTIME;
+    } else {
+        const int64_t n4 = n_floats / 4;
+        const int grid = (int)std::min<int64_t>(8192, (n4 + 255) / 256);
+        k_copy_probe<<<grid, 256>>>((const float4*)a, (float4*)b, n4);   // warm up
+        (void)hipEventRecord(e0, nullptr);
+        for (int r = 0; r < reps; ++r) k_copy_probe<<<grid, 256>>>((const float4*)a, (float4*)b, n4);
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0.f;
+        if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            status = FH_ERR_RUNTIME;
+        *avg_ms = (double)ms / reps;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (status != FH_OK) g_error = "copy probe failed";
+    return status;
+}
 
 }  // extern "C"

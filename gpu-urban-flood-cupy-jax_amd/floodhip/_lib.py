@@ -26,6 +26,7 @@ class FhParams(C.Structure):
         ("precision", C.c_int32), ("use_graph", C.c_int32), ("steps_per_launch", C.c_int32),
         ("block_size", C.c_int32), ("xcd_remap", C.c_int32), ("device", C.c_int32),
         ("edges_per_thread", C.c_int32), ("pack_order", C.c_int32),
+        ("cell_order", C.c_int32),
     ]
 
 
@@ -73,6 +74,7 @@ SIGNATURES = {
     "fh_profile_kernels": (C.c_int, [_ctx, C.c_int32, C.c_int32, _i, C.c_char_p, _d, _d,
                                      C.POINTER(C.c_int64)]),
     "fh_device_bytes": (C.c_int64, [_ctx]),
+    "fh_copy_probe": (C.c_int, [C.c_int64, C.c_int32, _d]),
 }
 
 _LIB = None

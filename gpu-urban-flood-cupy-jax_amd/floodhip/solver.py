@@ -79,6 +79,7 @@ class SolverOptions(NamedTuple):
     device: int = 0
     edges_per_thread: int = 1
     pack_order: int = 0
+    cell_order: int = 1
     cfl: float = 0.85
     cfl_interval: int = 32
     reuse_factor: float = 0.8
@@ -117,6 +118,7 @@ class HipParitySolver:
         p.max_speed = float(opts.max_speed)
         p.edges_per_thread = int(opts.edges_per_thread)
         p.pack_order = int(opts.pack_order)
+        p.cell_order = int(opts.cell_order)
         self._ctx = self.lib.fh_create_batch(C.byref(p), int(members))
         if not self._ctx:
             msg = self.lib.fh_global_error().decode()

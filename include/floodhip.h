@@ -67,6 +67,8 @@ typedef struct {
     int32_t device;             /* HIP device ordinal */
     int32_t edges_per_thread;   /* 1, 2 or 4 edges per thread in the flux kernel */
     int32_t pack_order;         /* batch member order: 0 as given, 1 largest first, 2 large/small alternating */
+    int32_t cell_order;         /* device layout of cells and edges: 0 as given, 1 breadth-first (Cuthill-McKee)
+                                   locality order; results are returned in the caller's order either way */
 } fh_params;
 
 typedef struct {
@@ -180,6 +182,12 @@ int fh_profile_kernels(fh_ctx* ctx, int32_t steps, int32_t cap, int32_t* count,
 
 /* Size of the resident device footprint in bytes. */
 int64_t fh_device_bytes(const fh_ctx* ctx);
+
+/* Measurement helper (no counterpart in the reference): times `reps` launches
+ * of a plain streaming copy of n_floats floats (n_floats % 4 == 0) on device 0
+ * and returns the average milliseconds per launch.  Used as the reference
+ * point for HBM byte counters and achievable bandwidth. */
+int fh_copy_probe(int64_t n_floats, int32_t reps, double* avg_ms);
 
 #ifdef __cplusplus
 }
