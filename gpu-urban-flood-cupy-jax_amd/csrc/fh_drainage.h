@@ -28,14 +28,19 @@ struct DrainDev {
     real* Vp;               // node volume after the inlet exchange of this step
     real *sur_factor, *overflow, *fallback, *wsum;
     const int* explicit_;  // node has valid entry-table rows
-    const int *ni_ptr, *ni_idx;        // node -> inlets
+    // node -> inlets, ascending inlet ("slots"): the inlet's cell and surcharge
+    // limit, and per step the scaled surcharge and net exchange (capture - surcharge)
+    const int *ni_ptr, *ni_cell;
+    const real* ni_surq;
+    real *ni_sur, *ni_net;
     const int *nl_ptr, *nl_ent;        // node -> incident links, ascending link, (l << 1) | is_to
-    const int *ne_ptr, *ne_idx;        // node -> valid entries
-    // inlets
-    const int *icell, *inode;
-    const real *capq, *surq;
-    real *sur, *cap;
-    const int *ic_cells, *ic_ptr, *ic_idx;  // inlet cells -> inlets
+    // node -> incident links, 'from' entries then 'to' entries (each ascending), with the far node
+    const int *nl2_ptr, *nl2_ent, *nl2_oth;
+    const int* ne_ptr;                 // node -> valid entries
+    const real* ne_sw;                 //   their weights
+    // inlet cells -> inlets (ascending): node, capture limit, node slot
+    const int *ic_cells, *ic_ptr, *ic_node, *ic_slot;
+    const real* ic_capq;
     // links
     const int *from, *to, *geom;
     const real *len, *man, *lcap, *linv, *D, *minor, *rw, *rh;
@@ -44,8 +49,9 @@ struct DrainDev {
     const int *snode;
     const real* sw;
     const int* snorm;        // [members] surcharge weights already normalized
-    // return cells
-    const int *rc_cells, *rc_eptr, *rc_eidx, *rc_fptr, *rc_fidx, *rc_rptr, *rc_ridx;
+    // return cells: entries (node, weight), fallback nodes, recharge rows
+    const int *rc_cells, *rc_eptr, *rc_enode, *rc_fptr, *rc_fidx, *rc_rptr, *rc_ridx;
+    const real* rc_esw;
     // recharge rows
     const real *rS, *rcc, *rrc, *rstop;
     real* rstore;
@@ -125,73 +131,73 @@ __global__ void __launch_bounds__(256) k_external(Ctl* __restrict__ ctls, KP<rea
     accumulate_slot<1>(part, b, v);
 }
 
+// The kernels below load what does not depend on the member's control block
+// (the thread's own index rows) before testing whether the member's step is
+// active: the grid covers exactly the padded index space, so those loads stay
+// in bounds, and the gather chains start one memory round trip earlier.
+
 // Inlet exchange, pass A (one thread per node): surcharge requests of the node's
 // inlets and the node's donor factor  min(1, V / sum(requests)).
 template <typename real>
 __global__ void __launch_bounds__(256) k_inlet_node_factor(const Ctl* __restrict__ ctls, KP<real> p,
         DrainDev<real> d, const typename V4<real>::type* __restrict__ state) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k0 = d.ni_ptr[n], k1 = d.ni_ptr[n + 1];
     const int mem = d.seg_node.blk_member[blockIdx.x];
+    if (k0 == k1) return;
+    const real vol = d.V[n];
+    const real inv = d.inv[n], As = d.As[n];
     const Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= d.seg_node.mem_end[mem]) return;
-    const int k0 = d.ni_ptr[n], k1 = d.ni_ptr[n + 1];
-    if (k0 == k1) return;
     const real dt = (real)ctl->dt_real;
-    const real vol = d.V[n];
-    const real nh = node_head(d, n, vol);
+    const real nh = inv + vmax(vol, real(0)) / vmax(As, real(1.0e-12));
     real total = real(0);
     for (int k = k0; k < k1; ++k) {
-        const int i = d.ni_idx[k];
-        const int c = d.icell[i];
-        const typename V4<real>::type s = state[c];
+        const typename V4<real>::type s = state[d.ni_cell[k]];
         const real q = orifice_q(p, (s.w + s.x) - nh);
-        const real rs = vmin(vmax(-q, real(0)) * dt, vmax(d.surq[i], real(0)) * dt);
-        d.sur[i] = rs;
+        const real rs = vmin(vmax(-q, real(0)) * dt, vmax(d.ni_surq[k], real(0)) * dt);
+        d.ni_sur[k] = rs;
         total = total + rs;
     }
     const real factor = vmin(real(1), vol / vmax(total, real(1.0e-12)));
-    for (int k = k0; k < k1; ++k) {
-        const int i = d.ni_idx[k];
-        d.sur[i] = d.sur[i] * factor;
-    }
+    for (int k = k0; k < k1; ++k) d.ni_sur[k] = d.ni_sur[k] * factor;
 }
 
 // Inlet exchange, pass B (one thread per inlet cell): capture requests, the
 // cell's donor factor min(1, h A / sum), captured volumes, and the depth update
-// h = max(h + sum((sur - cap) / A), 0).
+// h = max(h + sum((sur - cap) / A), 0).  Each inlet's net exchange
+// (capture - surcharge) goes to its node slot for the node volume update.
 template <typename real>
 __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
         typename V4<real>::type* __restrict__ state, const real* __restrict__ area, double* __restrict__ part) {
     const int b = blockIdx.x;
+    const int j = b * blockDim.x + threadIdx.x;
+    const int c = d.ic_cells[j];
+    const int k0 = d.ic_ptr[j], k1 = d.ic_ptr[j + 1];
     const int mem = d.seg_icell.blk_member[b];
     Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
-    const int j = b * blockDim.x + threadIdx.x;
     double cap_sum = 0.0, sur_sum = 0.0;
     if (j < d.seg_icell.mem_end[mem]) {
-        const int c = d.ic_cells[j];
         const real dt = (real)ctl->dt_real;
         typename V4<real>::type s = state[c];
         const real A = area[c];
         const real sh = s.w + s.x;
-        const int k0 = d.ic_ptr[j], k1 = d.ic_ptr[j + 1];
         real total = real(0);
         for (int k = k0; k < k1; ++k) {
-            const int i = d.ic_idx[k];
-            const int n = d.inode[i];
+            const int n = d.ic_node[k];
             const real q = orifice_q(p, sh - node_head(d, n, d.V[n]));
-            const real rc = vmin(vmax(q, real(0)) * dt, vmax(d.capq[i], real(0)) * dt);
-            d.cap[i] = rc;
+            const real rc = vmin(vmax(q, real(0)) * dt, vmax(d.ic_capq[k], real(0)) * dt);
+            d.ni_net[d.ic_slot[k]] = rc;          // request, scaled below
             total = total + rc;
         }
         const real factor = vmin(real(1), (s.x * A) / vmax(total, real(1.0e-12)));
         real acc = real(0);
         for (int k = k0; k < k1; ++k) {
-            const int i = d.ic_idx[k];
-            const real cp = d.cap[i] * factor;
-            d.cap[i] = cp;
-            const real sr = d.sur[i];
+            const int slot = d.ic_slot[k];
+            const real cp = d.ni_net[slot] * factor;
+            const real sr = d.ni_sur[slot];
+            d.ni_net[slot] = cp - sr;
             acc = acc + (sr - cp) / A;
             cap_sum += (double)cp;
             sur_sum += (double)sr;
@@ -208,13 +214,10 @@ __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<r
 // Node volume after the inlet exchange (v1_parity.py:292-298):
 // max(V + sum over the node's inlets of (capture - surcharge), 0), for every node.
 template <typename real>
-__device__ __forceinline__ real post_inlet_volume(const DrainDev<real>& d, int n) {
+__device__ __forceinline__ real post_inlet_volume(const DrainDev<real>& d, int k0, int k1, real vol) {
     real acc = real(0);
-    for (int k = d.ni_ptr[n]; k < d.ni_ptr[n + 1]; ++k) {
-        const int i = d.ni_idx[k];
-        acc = acc + (d.cap[i] - d.sur[i]);
-    }
-    return vmax(d.V[n] + acc, real(0));
+    for (int k = k0; k < k1; ++k) acc = acc + d.ni_net[k];
+    return vmax(vol + acc, real(0));
 }
 
 template <typename real>
@@ -228,13 +231,11 @@ __device__ __forceinline__ void circular_theta(real depth, real diameter, real& 
 
 // Pipe link momentum (v1_parity.py:324-347): semi-implicit Saint-Venant update
 // with Manning friction and minor losses, clipped to capacity; returns the
-// signed requested transfer Q dt.  Va / Vb are the end-node volumes.
+// signed requested transfer Q dt.  ha / hb are the end-node heads.
 template <typename real>
-__device__ __forceinline__ real link_request(const KP<real>& p, const DrainDev<real>& d, int l, real Va,
-                                             real Vb, real dt) {
+__device__ __forceinline__ real link_request(const KP<real>& p, const DrainDev<real>& d, int l, real ha,
+                                             real hb, real dt) {
     const real zero = real(0), g = p.g;
-    const int a = d.from[l], b = d.to[l];
-    const real ha = node_head(d, a, Va), hb = node_head(d, b, Vb);
     const real depth = vmax(real(0.5) * (ha + hb) - d.linv[l], zero);
     const real Draw = d.D[l];
     const real Dm = vmax(Draw, real(1.0e-12));
@@ -272,16 +273,18 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const DrainDev<r
 // post-inlet volumes of the link's two end nodes (v1_parity.py:324-347).
 template <typename real>
 __global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = d.from[l], b = d.to[l];
     const int mem = d.seg_link.blk_member[blockIdx.x];
+    const int a0 = d.ni_ptr[a], a1 = d.ni_ptr[a + 1], b0 = d.ni_ptr[b], b1 = d.ni_ptr[b + 1];
+    const real Va = d.V[a], Vb = d.V[b];
     const Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= d.seg_link.mem_end[mem]) return;
     const real dt = (real)ctl->dt_real;
-    const int a = d.from[l], b = d.to[l];
-    const real va = post_inlet_volume(d, a);
-    const real vb = a == b ? va : post_inlet_volume(d, b);
-    d.req[l] = link_request(p, d, l, va, vb, dt);
+    const real va = post_inlet_volume(d, a0, a1, Va);
+    const real vb = a == b ? va : post_inlet_volume(d, b0, b1, Vb);
+    d.req[l] = link_request(p, d, l, node_head(d, a, va), node_head(d, b, vb), dt);
 }
 
 // Donor factors (one thread per node): the node's post-inlet volume and
@@ -289,19 +292,31 @@ __global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<r
 // order) -- v1_parity.py:348-352.
 template <typename real>
 __global__ void __launch_bounds__(256) k_node_factor(const Ctl* __restrict__ ctls, DrainDev<real> d) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = d.ni_ptr[n], i1 = d.ni_ptr[n + 1];
+    const int k0 = d.nl_ptr[n], k1 = d.nl_ptr[n + 1];
+    const real vol = d.V[n];
     const int mem = d.seg_node.blk_member[blockIdx.x];
     if (!ctls[mem].active) return;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= d.seg_node.mem_end[mem]) return;
-    const real vn = post_inlet_volume(d, n);
+    const real vn = post_inlet_volume(d, i0, i1, vol);
     d.Vp[n] = vn;
     real total = real(0);
-    for (int k = d.nl_ptr[n]; k < d.nl_ptr[n + 1]; ++k) {
-        const int en = d.nl_ent[k];
-        const real r = d.req[en >> 1];
-        const bool is_to = en & 1;
-        const bool positive = r >= real(0);
-        if (positive != is_to) total = total + fabs(r);   // donor = from if positive else to
+    constexpr int KB = 4;
+    for (int kb = k0; kb < k1; kb += KB) {
+        int en[KB];
+        real r[KB];
+#pragma unroll
+        for (int u = 0; u < KB; ++u) en[u] = kb + u < k1 ? d.nl_ent[kb + u] : 0;
+#pragma unroll
+        for (int u = 0; u < KB; ++u) r[u] = kb + u < k1 ? d.req[en[u] >> 1] : real(0);
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            if (kb + u >= k1) break;
+            const bool is_to = en[u] & 1;
+            const bool positive = r[u] >= real(0);
+            if (positive != is_to) total = total + fabs(r[u]);   // donor = from if positive else to
+        }
     }
     d.sur_factor[n] = vmin(real(1), vn / vmax(total, real(1.0e-12)));
 }
@@ -314,31 +329,42 @@ template <typename real>
 __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
                                                     double* __restrict__ part) {
     const int b = blockIdx.x;
+    const int n = b * blockDim.x + threadIdx.x;
+    const int k0 = d.nl2_ptr[n], k1 = d.nl2_ptr[n + 1];
+    const int e0 = d.ne_ptr[n], e1 = d.ne_ptr[n + 1];
     const int mem = d.seg_node.blk_member[b];
     Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
-    const int n = b * blockDim.x + threadIdx.x;
     const real zero = real(0);
     double out_sum = 0.0, total_sum = 0.0, unret_sum = 0.0;
     if (n < d.seg_node.mem_end[mem]) {
         const real dt = (real)ctl->dt_real;
         real vol = d.Vp[n];
+        const real f_self = d.sur_factor[n];
         {
-            const int k0 = d.nl_ptr[n], k1 = d.nl_ptr[n + 1];
             real delta = zero;
-            for (int pass = 0; pass < 2; ++pass) {
-                for (int k = k0; k < k1; ++k) {
-                    const int en = d.nl_ent[k];
-                    const int is_to = en & 1;
-                    if (is_to != pass) continue;
-                    const int l = en >> 1;
-                    const real r = d.req[l];
-                    const bool positive = r >= zero;
-                    const int donor = positive ? d.from[l] : d.to[l];
-                    const real T = (positive ? real(1) : real(-1)) * fabs(r) * d.sur_factor[donor];
+            constexpr int KB = 4;
+            for (int kb = k0; kb < k1; kb += KB) {
+                int en[KB];
+                real r[KB], fo[KB];
+#pragma unroll
+                for (int u = 0; u < KB; ++u) en[u] = kb + u < k1 ? d.nl2_ent[kb + u] : 0;
+#pragma unroll
+                for (int u = 0; u < KB; ++u) {
+                    r[u] = kb + u < k1 ? d.req[en[u] >> 1] : zero;
+                    fo[u] = kb + u < k1 ? d.sur_factor[d.nl2_oth[kb + u]] : zero;
+                }
+#pragma unroll
+                for (int u = 0; u < KB; ++u) {
+                    if (kb + u >= k1) break;
+                    const int is_to = en[u] & 1;
+                    const bool positive = r[u] >= zero;
+                    // donor = from if positive else to; this node is 'to' when is_to
+                    const real f = (positive != (is_to != 0)) ? f_self : fo[u];
+                    const real T = (positive ? real(1) : real(-1)) * fabs(r[u]) * f;
                     if (!is_to) {
                         delta = delta + (-T);
-                        d.Q[l] = T / vmax(dt, real(1.0e-12));
+                        d.Q[en[u] >> 1] = T / vmax(dt, real(1.0e-12));
                     } else {
                         delta = delta + T;
                     }
@@ -359,10 +385,9 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         real returned = zero;
         real wsum = zero;
         if (d.S > 0) {
-            const int k0 = d.ne_ptr[n], k1 = d.ne_ptr[n + 1];
-            for (int k = k0; k < k1; ++k) wsum = wsum + vmax(d.sw[d.ne_idx[k]], zero);
-            for (int k = k0; k < k1; ++k) {
-                const real raw = vmax(d.sw[d.ne_idx[k]], zero);
+            for (int k = e0; k < e1; ++k) wsum = wsum + vmax(d.ne_sw[k], zero);
+            for (int k = e0; k < e1; ++k) {
+                const real raw = vmax(d.ne_sw[k], zero);
                 const real norm = d.snorm[mem] ? raw : raw / vmax(wsum, real(1.0e-12));
                 returned = returned + over * norm;
             }
@@ -392,23 +417,23 @@ template <typename real>
 __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
         typename V4<real>::type* __restrict__ state, const real* __restrict__ area, double* __restrict__ part) {
     const int b = blockIdx.x;
+    const int j = b * blockDim.x + threadIdx.x;
+    const int c = d.rc_cells[j];
+    const int ke0 = d.rc_eptr[j], ke1 = d.rc_eptr[j + 1];
     const int mem = d.seg_ret.blk_member[b];
     Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
-    const int j = b * blockDim.x + threadIdx.x;
     const real zero = real(0);
     double cap_sum = 0.0, loss_sum = 0.0;
     if (j < d.seg_ret.mem_end[mem]) {
-        const int c = d.rc_cells[j];
         const real dt = (real)ctl->dt_real;
         const real A = area[c];
         real h = state[c].x;
         if (d.S > 0) {
             real acc = zero;
-            for (int k = d.rc_eptr[j]; k < d.rc_eptr[j + 1]; ++k) {
-                const int s = d.rc_eidx[k];
-                const int n = d.snode[s];
-                const real raw = vmax(d.sw[s], zero);
+            for (int k = ke0; k < ke1; ++k) {
+                const int n = d.rc_enode[k];
+                const real raw = vmax(d.rc_esw[k], zero);
                 const real norm = d.snorm[mem] ? raw : raw / vmax(d.wsum[n], real(1.0e-12));
                 acc = acc + (d.overflow[n] * norm) / A;
             }

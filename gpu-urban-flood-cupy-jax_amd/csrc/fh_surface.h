@@ -143,15 +143,19 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
         }
         vmin_ = (double)mn;
     }
-    vmin_ = block_min(vmin_, lds);
-    if (eval && threadIdx.x == 0) {
-        // non-negative doubles order like their bit patterns: one atomic min per block
-        __hip_atomic_fetch_min(&ctl->cfl_bits, (unsigned long long)__double_as_longlong(vmin_), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+    if (eval) {
+        vmin_ = block_min(vmin_, lds);
+        if (threadIdx.x == 0) {
+            // non-negative doubles order like their bit patterns: one atomic min per block
+            __hip_atomic_fetch_min(&ctl->cfl_bits, (unsigned long long)__double_as_longlong(vmin_),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
+        if (threadIdx.x != 0) return;
+        __hip_atomic_store(&ctl->counters[C_BEGIN], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (b != blk0 || threadIdx.x != 0) {
+        return;   // a step that reuses the cached CFL needs only the scalar part below
     }
-    if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
-    if (threadIdx.x != 0) return;
-    __hip_atomic_store(&ctl->counters[C_BEGIN], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     double gmin = INFINITY;
     if (eval) {
         const unsigned long long bits = __hip_atomic_load(&ctl->cfl_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

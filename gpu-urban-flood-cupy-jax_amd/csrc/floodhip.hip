@@ -1002,19 +1002,68 @@ public:
         std::vector<real> zN(NT, 0);
         d_Vp.set(zN, s);
         d_sfac.set(zN, s); d_over.set(zN, s); d_fb.set(zN, s); d_wsum.set(zN, s);
-        d_ni_ptr.set(ni_ptr, s); d_ni_idx.set(ni_idx, s); d_nl_ptr.set(nl_ptr, s); d_nl_ent.set(nl_ent, s);
-        d_ne_ptr.set(ne_ptr, s); d_ne_idx.set(ne_idx, s);
-        d_icell.set(icell, s); d_inode.set(inode, s); d_capq.set(capq, s); d_surq.set(surq, s);
-        std::vector<real> zI(It, 0);
-        d_sur.set(zI, s); d_cap.set(zI, s);
-        d_ic_cells.set(ic_cells, s); d_ic_ptr.set(ic_ptr, s); d_ic_idx.set(ic_idx, s);
+        // composed index maps: each kernel reads per-slot data in the order it consumes it
+        {
+            const int nslots = (int)ni_idx.size();
+            std::vector<int> ni_cell(nslots), slot_of_inlet(std::max(It, 0), -1);
+            std::vector<real> ni_surq(nslots);
+            for (int k = 0; k < nslots; ++k) {
+                const int i = ni_idx[k];
+                ni_cell[k] = icell[i];
+                ni_surq[k] = surq[i];
+                slot_of_inlet[i] = k;
+            }
+            const int nic = (int)ic_idx.size();
+            std::vector<int> ic_node(nic), ic_slot(nic);
+            std::vector<real> ic_capq(nic);
+            for (int k = 0; k < nic; ++k) {
+                const int i = ic_idx[k];
+                ic_node[k] = inode[i];
+                ic_capq[k] = capq[i];
+                ic_slot[k] = slot_of_inlet[i];
+            }
+            // node -> links: 'from' entries then 'to' entries, each ascending, with the far node
+            std::vector<int> nl2_ptr(NT + 1, 0), nl2_ent(nl_ent.size()), nl2_oth(nl_ent.size());
+            int w = 0;
+            for (int n = 0; n < NT; ++n) {
+                nl2_ptr[n] = w;
+                for (int pass = 0; pass < 2; ++pass)
+                    for (int k = nl_ptr[n]; k < nl_ptr[n + 1]; ++k) {
+                        const int en = nl_ent[k];
+                        if ((en & 1) != pass) continue;
+                        const int l = en >> 1;
+                        nl2_ent[w] = en;
+                        nl2_oth[w] = pass ? from[l] : to[l];
+                        ++w;
+                    }
+            }
+            nl2_ptr[NT] = w;
+            std::vector<real> ne_sw(ne_idx.size());
+            for (size_t k = 0; k < ne_idx.size(); ++k) ne_sw[k] = sw[ne_idx[k]];
+            std::vector<int> rc_enode(rc_eidx.size());
+            std::vector<real> rc_esw(rc_eidx.size());
+            for (size_t k = 0; k < rc_eidx.size(); ++k) {
+                rc_enode[k] = snode[rc_eidx[k]];
+                rc_esw[k] = sw[rc_eidx[k]];
+            }
+            d_ni_cell.set(ni_cell, s); d_ni_surq.set(ni_surq, s);
+            std::vector<real> zS((size_t)nslots, 0);
+            d_ni_sur.set(zS, s); d_ni_net.set(zS, s);
+            d_ic_node.set(ic_node, s); d_ic_slot.set(ic_slot, s); d_ic_capq.set(ic_capq, s);
+            d_nl2_ptr.set(nl2_ptr, s); d_nl2_ent.set(nl2_ent, s); d_nl2_oth.set(nl2_oth, s);
+            d_ne_sw.set(ne_sw, s);
+            d_rc_enode.set(rc_enode, s); d_rc_esw.set(rc_esw, s);
+        }
+        d_ni_ptr.set(ni_ptr, s); d_nl_ptr.set(nl_ptr, s); d_nl_ent.set(nl_ent, s);
+        d_ne_ptr.set(ne_ptr, s);
+        d_ic_cells.set(ic_cells, s); d_ic_ptr.set(ic_ptr, s);
         d_from.set(from, s); d_to.set(to, s); d_geom.set(geom, s); d_llen.set(llen, s); d_man.set(man, s);
         d_lcap.set(lcap, s); d_linv.set(linv, s); d_D.set(D, s); d_minor.set(minor, s); d_rw.set(rw, s);
         d_rh.set(rh, s); d_Q.set(Q, s);
         std::vector<real> zL(LT, 0);
         d_req.set(zL, s);
         d_snode.set(snode, s); d_sw.set(sw, s); d_snorm.set(by_id(snorm, order), s);
-        d_rc_cells.set(rc_cells, s); d_rc_eptr.set(rc_eptr, s); d_rc_eidx.set(rc_eidx, s);
+        d_rc_cells.set(rc_cells, s); d_rc_eptr.set(rc_eptr, s);
         d_rc_fptr.set(rc_fptr, s); d_rc_fidx.set(rc_fidx, s); d_rc_rptr.set(rc_rptr, s); d_rc_ridx.set(rc_ridx, s);
         d_rS.set(rS, s); d_rcc.set(rcc, s); d_rrc.set(rrc, s); d_rstop.set(rstop, s); d_rstore.set(rstore, s);
         d_ext_q.set(q_all, s); d_rcv_qbase.set(rcv_qbase, s); d_rcv_stride.set(rcv_stride, s);
@@ -1038,14 +1087,17 @@ public:
         d.o_cap = d_ocap.p; d.nsc = d_nsc.p; d.V = d_V.p; d.Vp = d_Vp.p; d.sur_factor = d_sfac.p;
         d.overflow = d_over.p;
         d.fallback = d_fb.p; d.wsum = d_wsum.p; d.explicit_ = d_expl.p;
-        d.ni_ptr = d_ni_ptr.p; d.ni_idx = d_ni_idx.p; d.nl_ptr = d_nl_ptr.p; d.nl_ent = d_nl_ent.p;
-        d.ne_ptr = d_ne_ptr.p; d.ne_idx = d_ne_idx.p;
-        d.icell = d_icell.p; d.inode = d_inode.p; d.capq = d_capq.p; d.surq = d_surq.p; d.sur = d_sur.p;
-        d.cap = d_cap.p; d.ic_cells = d_ic_cells.p; d.ic_ptr = d_ic_ptr.p; d.ic_idx = d_ic_idx.p;
+        d.ni_ptr = d_ni_ptr.p; d.ni_cell = d_ni_cell.p; d.ni_surq = d_ni_surq.p; d.ni_sur = d_ni_sur.p;
+        d.ni_net = d_ni_net.p; d.nl_ptr = d_nl_ptr.p; d.nl_ent = d_nl_ent.p;
+        d.nl2_ptr = d_nl2_ptr.p; d.nl2_ent = d_nl2_ent.p; d.nl2_oth = d_nl2_oth.p;
+        d.ne_ptr = d_ne_ptr.p; d.ne_sw = d_ne_sw.p;
+        d.ic_cells = d_ic_cells.p; d.ic_ptr = d_ic_ptr.p; d.ic_node = d_ic_node.p; d.ic_slot = d_ic_slot.p;
+        d.ic_capq = d_ic_capq.p;
         d.from = d_from.p; d.to = d_to.p; d.geom = d_geom.p; d.len = d_llen.p; d.man = d_man.p; d.lcap = d_lcap.p;
         d.linv = d_linv.p; d.D = d_D.p; d.minor = d_minor.p; d.rw = d_rw.p; d.rh = d_rh.p; d.Q = d_Q.p;
         d.req = d_req.p; d.snode = d_snode.p; d.sw = d_sw.p; d.snorm = d_snorm.p;
-        d.rc_cells = d_rc_cells.p; d.rc_eptr = d_rc_eptr.p; d.rc_eidx = d_rc_eidx.p; d.rc_fptr = d_rc_fptr.p;
+        d.rc_cells = d_rc_cells.p; d.rc_eptr = d_rc_eptr.p; d.rc_enode = d_rc_enode.p; d.rc_esw = d_rc_esw.p;
+        d.rc_fptr = d_rc_fptr.p;
         d.rc_fidx = d_rc_fidx.p; d.rc_rptr = d_rc_rptr.p; d.rc_ridx = d_rc_ridx.p;
         d.rS = d_rS.p; d.rcc = d_rcc.p; d.rrc = d_rrc.p; d.rstop = d_rstop.p; d.rstore = d_rstore.p;
         d.ext_q = d_ext_q.p; d.rcv_qbase = d_rcv_qbase.p; d.rcv_stride = d_rcv_stride.p;
@@ -1332,7 +1384,8 @@ public:
         b += d_rec.bytes() + d_recR.bytes() + d_state.bytes() + d_wet.bytes() + d_comp.bytes();
         b += d_area.bytes() + d_n2.bytes() + d_lc.bytes() + d_rainm.bytes() + d_f0.bytes() + d_fc.bytes() + d_k.bytes();
         b += d_ce0.bytes() + d_ce1.bytes() + d_ce2.bytes();
-        b += d_inv.bytes() * 12 + d_ni_idx.bytes() + d_nl_ent.bytes() + d_icell.bytes() * 6 + d_from.bytes() * 16;
+        b += d_inv.bytes() * 12 + d_ni_cell.bytes() * 4 + d_nl_ent.bytes() * 3 + d_ic_node.bytes() * 3 +
+             d_from.bytes() * 16;
         return b;
     }
 
@@ -1360,14 +1413,15 @@ public:
     DevArray<float> d_rain_s, d_rain_e, d_rain_r, d_ext_t, d_ext_q, d_ext_ov;
     // drainage
     DevArray<real> d_inv, d_As, d_dmax, d_oa, d_oc, d_otw, d_ocap, d_V, d_Vp, d_sfac, d_over, d_fb, d_wsum;
-    DevArray<int> d_nsc, d_expl, d_ni_ptr, d_ni_idx, d_nl_ptr, d_nl_ent, d_ne_ptr, d_ne_idx;
-    DevArray<int> d_icell, d_inode, d_ic_cells, d_ic_ptr, d_ic_idx;
-    DevArray<real> d_capq, d_surq, d_sur, d_cap;
+    DevArray<int> d_nsc, d_expl, d_ni_ptr, d_ni_cell, d_nl_ptr, d_nl_ent, d_nl2_ptr, d_nl2_ent, d_nl2_oth, d_ne_ptr;
+    DevArray<int> d_ic_cells, d_ic_ptr, d_ic_node, d_ic_slot;
+    DevArray<real> d_ni_surq, d_ni_sur, d_ni_net, d_ic_capq, d_ne_sw;
     DevArray<int> d_from, d_to, d_geom;
     DevArray<real> d_llen, d_man, d_lcap, d_linv, d_D, d_minor, d_rw, d_rh, d_Q, d_req;
     DevArray<int> d_snode, d_snorm;
     DevArray<real> d_sw;
-    DevArray<int> d_rc_cells, d_rc_eptr, d_rc_eidx, d_rc_fptr, d_rc_fidx, d_rc_rptr, d_rc_ridx;
+    DevArray<int> d_rc_cells, d_rc_eptr, d_rc_enode, d_rc_fptr, d_rc_fidx, d_rc_rptr, d_rc_ridx;
+    DevArray<real> d_rc_esw;
     DevArray<real> d_rS, d_rcc, d_rrc, d_rstop, d_rstore;
     DevArray<int> d_rcv_qbase, d_rcv_stride, d_xc_cells, d_xc_ptr, d_xc_idx;
     fh::DrainDev<real> dd{};
