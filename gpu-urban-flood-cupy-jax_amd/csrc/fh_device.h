@@ -111,6 +111,16 @@ __device__ __forceinline__ T wave_min(T v) {
     return v;
 }
 
+// Compiler-only memory barrier: loads written above it are issued above it.
+// Kernels use it to start their gathers before testing the member's control
+// block (a test the compiler would otherwise place first, sinking the loads
+// into the taken path).  It emits no instruction and waits for nothing.
+#define FH_ISSUE_LOADS() asm volatile("" ::: "memory")
+// The same for loads through __restrict__ kernel arguments, which the compiler
+// would otherwise move across the barrier: naming the pointers makes the
+// barrier a possible writer of their memory.
+#define FH_ISSUE_LOADS_OF(...) asm volatile("" ::__VA_ARGS__ : "memory")
+
 // Deterministic block sum of N values (fixed butterfly + fixed wave order).
 // Result valid in thread 0.  lds must hold N*16 entries.
 template <int N, typename T>
@@ -195,6 +205,25 @@ __device__ __forceinline__ void accumulate_slot(double* acc, int slot, const dou
         for (int k = 0; k < N; ++k) acc[(size_t)slot * N + k] += v[k];
     }
 }
+
+// accumulate_slot with the slot's old value loaded at the start of the kernel,
+// so the final read-modify-write costs no memory round trip (same sum: old + v).
+template <int N>
+struct SlotAcc {
+    double old[N];
+    __device__ __forceinline__ void prefetch(const double* acc, int slot) {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) old[k] = acc[(size_t)slot * N + k];
+        }
+    }
+    __device__ __forceinline__ void commit(double* acc, int slot, const double (&v)[N]) const {
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc[(size_t)slot * N + k] = old[k] + v[k];
+        }
+    }
+};
 
 // Publish one block's partial record (agent-scope stores: visible to the last
 // block wherever it runs, across XCD L2s).

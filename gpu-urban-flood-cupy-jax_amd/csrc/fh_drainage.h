@@ -65,8 +65,13 @@ struct DrainDev {
 };
 
 template <typename real>
+__device__ __forceinline__ real head_of(real inv, real As, real vol) {
+    return inv + vmax(vol, real(0)) / vmax(As, real(1.0e-12));
+}
+
+template <typename real>
 __device__ __forceinline__ real node_head(const DrainDev<real>& d, int n, real vol) {
-    return d.inv[n] + vmax(vol, real(0)) / vmax(d.As[n], real(1.0e-12));
+    return head_of(d.inv[n], d.As[n], vol);
 }
 
 template <typename real>
@@ -74,6 +79,11 @@ __device__ __forceinline__ real orifice_q(const KP<real>& p, real dh) {
     const real sgn = dh > real(0) ? real(1) : (dh < real(0) ? real(-1) : real(0));
     return p.capture_q * sqrt(p.two_g * fabs(dh)) * sgn;
 }
+
+// Index of an entry loaded ahead of its row test: the entry itself when the
+// row has it, else entry 0 (every table holds at least one element); values
+// loaded for a missing entry are never used.
+__device__ __forceinline__ int ahead(int k, int k1) { return k < k1 ? k : 0; }
 
 // commit helper used by the last block of each ledger kernel
 template <int NT>
@@ -141,21 +151,32 @@ __global__ void __launch_bounds__(256) k_external(Ctl* __restrict__ ctls, KP<rea
 template <typename real>
 __global__ void __launch_bounds__(256) k_inlet_node_factor(const Ctl* __restrict__ ctls, KP<real> p,
         DrainDev<real> d, const typename V4<real>::type* __restrict__ state) {
+    typedef typename V4<real>::type V;
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k0 = d.ni_ptr[n], k1 = d.ni_ptr[n + 1];
     const int mem = d.seg_node.blk_member[blockIdx.x];
+    const int k0 = d.ni_ptr[n], k1 = d.ni_ptr[n + 1];
     if (k0 == k1) return;
     const real vol = d.V[n];
     const real inv = d.inv[n], As = d.As[n];
+    const real sq0 = d.ni_surq[k0];                 // first inlet, loaded ahead
+    const V s0 = state[d.ni_cell[k0]];
+    FH_ISSUE_LOADS_OF("s"(state));
     const Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
     const real dt = (real)ctl->dt_real;
-    const real nh = inv + vmax(vol, real(0)) / vmax(As, real(1.0e-12));
-    real total = real(0);
-    for (int k = k0; k < k1; ++k) {
-        const typename V4<real>::type s = state[d.ni_cell[k]];
+    const real nh = head_of(inv, As, vol);
+    auto request = [&](const V& s, real sq) {
         const real q = orifice_q(p, (s.w + s.x) - nh);
-        const real rs = vmin(vmax(-q, real(0)) * dt, vmax(d.ni_surq[k], real(0)) * dt);
+        return vmin(vmax(-q, real(0)) * dt, vmax(sq, real(0)) * dt);
+    };
+    real total = real(0);
+    {
+        const real rs = request(s0, sq0);
+        d.ni_sur[k0] = rs;
+        total = total + rs;
+    }
+    for (int k = k0 + 1; k < k1; ++k) {
+        const real rs = request(state[d.ni_cell[k]], d.ni_surq[k]);
         d.ni_sur[k] = rs;
         total = total + rs;
     }
@@ -170,33 +191,53 @@ __global__ void __launch_bounds__(256) k_inlet_node_factor(const Ctl* __restrict
 template <typename real>
 __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
         typename V4<real>::type* __restrict__ state, const real* __restrict__ area, double* __restrict__ part) {
+    typedef typename V4<real>::type V;
     const int b = blockIdx.x;
     const int j = b * blockDim.x + threadIdx.x;
+    const int mem = d.seg_icell.blk_member[b];
     const int c = d.ic_cells[j];
     const int k0 = d.ic_ptr[j], k1 = d.ic_ptr[j + 1];
-    const int mem = d.seg_icell.blk_member[b];
+    V s = state[c];
+    const real A = area[c];
+    // first inlet of the cell, loaded ahead (padding rows point at slot 0 data, never used)
+    const int kf = ahead(k0, k1);
+    const int n0 = d.ic_node[kf], slot0 = d.ic_slot[kf];
+    const real capq0 = d.ic_capq[kf];
+    const real V0 = d.V[n0], inv0 = d.inv[n0], As0 = d.As[n0], sur0 = d.ni_sur[slot0];
+    SlotAcc<2> acc2;
+    acc2.prefetch(part, b);
+    FH_ISSUE_LOADS_OF("s"(state), "s"(area), "s"(part));
     Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
     double cap_sum = 0.0, sur_sum = 0.0;
     if (j < d.seg_icell.mem_end[mem]) {
         const real dt = (real)ctl->dt_real;
-        typename V4<real>::type s = state[c];
-        const real A = area[c];
         const real sh = s.w + s.x;
         real total = real(0);
+        real rc0 = real(0);
         for (int k = k0; k < k1; ++k) {
-            const int n = d.ic_node[k];
-            const real q = orifice_q(p, sh - node_head(d, n, d.V[n]));
-            const real rc = vmin(vmax(q, real(0)) * dt, vmax(d.ic_capq[k], real(0)) * dt);
-            d.ni_net[d.ic_slot[k]] = rc;          // request, scaled below
+            real nh;
+            real cq;
+            if (k == k0) {
+                nh = head_of(inv0, As0, V0);
+                cq = capq0;
+            } else {
+                const int n = d.ic_node[k];
+                nh = head_of(d.inv[n], d.As[n], d.V[n]);
+                cq = d.ic_capq[k];
+            }
+            const real q = orifice_q(p, sh - nh);
+            const real rc = vmin(vmax(q, real(0)) * dt, vmax(cq, real(0)) * dt);
+            if (k == k0) rc0 = rc;
+            else d.ni_net[d.ic_slot[k]] = rc;          // request, scaled below
             total = total + rc;
         }
         const real factor = vmin(real(1), (s.x * A) / vmax(total, real(1.0e-12)));
         real acc = real(0);
         for (int k = k0; k < k1; ++k) {
-            const int slot = d.ic_slot[k];
-            const real cp = d.ni_net[slot] * factor;
-            const real sr = d.ni_sur[slot];
+            const int slot = k == k0 ? slot0 : d.ic_slot[k];
+            const real cp = (k == k0 ? rc0 : d.ni_net[slot]) * factor;
+            const real sr = k == k0 ? sur0 : d.ni_sur[slot];
             d.ni_net[slot] = cp - sr;
             acc = acc + (sr - cp) / A;
             cap_sum += (double)cp;
@@ -208,7 +249,7 @@ __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<r
     __shared__ double lds[32];
     double v[2] = {cap_sum, sur_sum};
     block_sum<2>(v, lds);
-    accumulate_slot<2>(part, b, v);
+    acc2.commit(part, b, v);
 }
 
 // Node volume after the inlet exchange (v1_parity.py:292-298):
@@ -229,15 +270,30 @@ __device__ __forceinline__ void circular_theta(real depth, real diameter, real& 
     theta = real(2) * acos(x);
 }
 
+// Static data of one link, loaded up front.
+template <typename real>
+struct LinkData {
+    real linv, D, len, man, q0, minor, lcap, rw, rh;
+    int geom;
+};
+
+template <typename real>
+__device__ __forceinline__ LinkData<real> load_link(const DrainDev<real>& d, int l) {
+    LinkData<real> k;
+    k.linv = d.linv[l]; k.D = d.D[l]; k.len = d.len[l]; k.man = d.man[l]; k.q0 = d.Q[l];
+    k.minor = d.minor[l]; k.lcap = d.lcap[l]; k.rw = d.rw[l]; k.rh = d.rh[l]; k.geom = d.geom[l];
+    return k;
+}
+
 // Pipe link momentum (v1_parity.py:324-347): semi-implicit Saint-Venant update
 // with Manning friction and minor losses, clipped to capacity; returns the
 // signed requested transfer Q dt.  ha / hb are the end-node heads.
 template <typename real>
-__device__ __forceinline__ real link_request(const KP<real>& p, const DrainDev<real>& d, int l, real ha,
-                                             real hb, real dt) {
+__device__ __forceinline__ real link_request(const KP<real>& p, const LinkData<real>& k, real ha, real hb,
+                                             real dt) {
     const real zero = real(0), g = p.g;
-    const real depth = vmax(real(0.5) * (ha + hb) - d.linv[l], zero);
-    const real Draw = d.D[l];
+    const real depth = vmax(real(0.5) * (ha + hb) - k.linv, zero);
+    const real Draw = k.D;
     const real Dm = vmax(Draw, real(1.0e-12));
     const real slot_width = g * (real(0.25 * M_PI) * (Dm * Dm)) / p.slot_coef;
     real radius_c, theta;
@@ -248,23 +304,22 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const DrainDev<r
     real flow_area = circ_area + slot_width * vmax(depth - Draw, zero);
     const real perim = depth >= Dm ? real(M_PI) * Dm : radius_c * theta;
     real radius = circ_area / vmax(perim, real(1.0e-12));
-    if (d.geom[l] == 2) {
-        const real od = vmin(vmax(depth, zero), vmax(d.rh[l], real(1.0e-6)));
-        const real oa = vmax(d.rw[l], real(1.0e-6)) * od;
+    if (k.geom == 2) {
+        const real od = vmin(vmax(depth, zero), vmax(k.rh, real(1.0e-6)));
+        const real oa = vmax(k.rw, real(1.0e-6)) * od;
         flow_area = oa;
-        radius = oa / vmax(d.rw[l] + real(2) * od, real(1.0e-12));
+        radius = oa / vmax(k.rw + real(2) * od, real(1.0e-12));
     }
-    const real length = vmax(d.len[l], real(1.0e-6));
+    const real length = vmax(k.len, real(1.0e-6));
     const real area = vmax(flow_area, real(1.0e-12));
     radius = vmax(radius, real(1.0e-12));
     const real slope = (ha - hb) / length;
-    const real nm = vmax(d.man[l], zero);
-    const real q0 = d.Q[l];
-    const real absq = fabs(q0);
+    const real nm = vmax(k.man, zero);
+    const real absq = fabs(k.q0);
     const real friction = g * (nm * nm) * absq / (pow(radius, p.pow43) * area);
-    const real minor = vmax(d.minor[l], zero) * absq / (real(2) * length * area);
-    real cand = (q0 + g * area * dt * slope) / (real(1) + dt * (friction + minor));
-    const real cap = vmax(d.lcap[l], zero);
+    const real minor = vmax(k.minor, zero) * absq / (real(2) * length * area);
+    real cand = (k.q0 + g * area * dt * slope) / (real(1) + dt * (friction + minor));
+    const real cap = vmax(k.lcap, zero);
     cand = vmin(vmax(cand, -cap), cap);
     return cand * dt;
 }
@@ -274,17 +329,20 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const DrainDev<r
 template <typename real>
 __global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = d.from[l], b = d.to[l];
     const int mem = d.seg_link.blk_member[blockIdx.x];
+    const int a = d.from[l], b = d.to[l];
+    const LinkData<real> lk = load_link(d, l);
     const int a0 = d.ni_ptr[a], a1 = d.ni_ptr[a + 1], b0 = d.ni_ptr[b], b1 = d.ni_ptr[b + 1];
     const real Va = d.V[a], Vb = d.V[b];
+    const real inva = d.inv[a], Asa = d.As[a], invb = d.inv[b], Asb = d.As[b];
+    FH_ISSUE_LOADS();
     const Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
     if (l >= d.seg_link.mem_end[mem]) return;
     const real dt = (real)ctl->dt_real;
     const real va = post_inlet_volume(d, a0, a1, Va);
     const real vb = a == b ? va : post_inlet_volume(d, b0, b1, Vb);
-    d.req[l] = link_request(p, d, l, node_head(d, a, va), node_head(d, b, vb), dt);
+    d.req[l] = link_request(p, lk, head_of(inva, Asa, va), head_of(invb, Asb, vb), dt);
 }
 
 // Donor factors (one thread per node): the node's post-inlet volume and
@@ -293,23 +351,37 @@ __global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<r
 template <typename real>
 __global__ void __launch_bounds__(256) k_node_factor(const Ctl* __restrict__ ctls, DrainDev<real> d) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mem = d.seg_node.blk_member[blockIdx.x];
     const int i0 = d.ni_ptr[n], i1 = d.ni_ptr[n + 1];
     const int k0 = d.nl_ptr[n], k1 = d.nl_ptr[n + 1];
     const real vol = d.V[n];
-    const int mem = d.seg_node.blk_member[blockIdx.x];
+    // the first inlet slot and the first KB link entries, loaded ahead
+    constexpr int KB = 4;
+    const real net0 = d.ni_net[ahead(i0, i1)];
+    int en[KB];
+    real r[KB];
+#pragma unroll
+    for (int u = 0; u < KB; ++u) en[u] = d.nl_ent[ahead(k0 + u, k1)];
+#pragma unroll
+    for (int u = 0; u < KB; ++u) r[u] = d.req[en[u] >> 1];
+    FH_ISSUE_LOADS();
     if (!ctls[mem].active) return;
     if (n >= d.seg_node.mem_end[mem]) return;
-    const real vn = post_inlet_volume(d, i0, i1, vol);
+    real vn;
+    {
+        real acc = real(0);
+        for (int k = i0; k < i1; ++k) acc = acc + (k == i0 ? net0 : d.ni_net[k]);
+        vn = vmax(vol + acc, real(0));
+    }
     d.Vp[n] = vn;
     real total = real(0);
-    constexpr int KB = 4;
     for (int kb = k0; kb < k1; kb += KB) {
-        int en[KB];
-        real r[KB];
+        if (kb != k0) {
 #pragma unroll
-        for (int u = 0; u < KB; ++u) en[u] = kb + u < k1 ? d.nl_ent[kb + u] : 0;
+            for (int u = 0; u < KB; ++u) en[u] = kb + u < k1 ? d.nl_ent[kb + u] : 0;
 #pragma unroll
-        for (int u = 0; u < KB; ++u) r[u] = kb + u < k1 ? d.req[en[u] >> 1] : real(0);
+            for (int u = 0; u < KB; ++u) r[u] = kb + u < k1 ? d.req[en[u] >> 1] : real(0);
+        }
 #pragma unroll
         for (int u = 0; u < KB; ++u) {
             if (kb + u >= k1) break;
@@ -330,29 +402,46 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
                                                     double* __restrict__ part) {
     const int b = blockIdx.x;
     const int n = b * blockDim.x + threadIdx.x;
+    const int mem = d.seg_node.blk_member[b];
+    const real zero = real(0);
     const int k0 = d.nl2_ptr[n], k1 = d.nl2_ptr[n + 1];
     const int e0 = d.ne_ptr[n], e1 = d.ne_ptr[n + 1];
-    const int mem = d.seg_node.blk_member[b];
+    const real vp = d.Vp[n], f_self = d.sur_factor[n];
+    const real o_tw = d.o_tw[n], o_coef = d.o_coef[n], o_area = d.o_area[n], o_cap = d.o_cap[n];
+    const real inv = d.inv[n], As = d.As[n], dmax = d.dmax[n];
+    const int expl = d.explicit_[n], nsc = d.nsc[n];
+    // the first KB link entries and the first entry weight, loaded ahead
+    constexpr int KB = 4;
+    int en[KB];
+    real r[KB], fo[KB];
+#pragma unroll
+    for (int u = 0; u < KB; ++u) en[u] = d.nl2_ent[ahead(k0 + u, k1)];
+#pragma unroll
+    for (int u = 0; u < KB; ++u) {
+        r[u] = d.req[en[u] >> 1];
+        fo[u] = d.sur_factor[d.nl2_oth[ahead(k0 + u, k1)]];
+    }
+    const real sw0 = d.ne_sw[ahead(e0, e1)];
+    SlotAcc<3> acc3;
+    acc3.prefetch(part, b);
+    FH_ISSUE_LOADS_OF("s"(part));
     Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
-    const real zero = real(0);
     double out_sum = 0.0, total_sum = 0.0, unret_sum = 0.0;
     if (n < d.seg_node.mem_end[mem]) {
         const real dt = (real)ctl->dt_real;
-        real vol = d.Vp[n];
-        const real f_self = d.sur_factor[n];
+        real vol = vp;
         {
             real delta = zero;
-            constexpr int KB = 4;
             for (int kb = k0; kb < k1; kb += KB) {
-                int en[KB];
-                real r[KB], fo[KB];
+                if (kb != k0) {
 #pragma unroll
-                for (int u = 0; u < KB; ++u) en[u] = kb + u < k1 ? d.nl2_ent[kb + u] : 0;
+                    for (int u = 0; u < KB; ++u) en[u] = kb + u < k1 ? d.nl2_ent[kb + u] : 0;
 #pragma unroll
-                for (int u = 0; u < KB; ++u) {
-                    r[u] = kb + u < k1 ? d.req[en[u] >> 1] : zero;
-                    fo[u] = kb + u < k1 ? d.sur_factor[d.nl2_oth[kb + u]] : zero;
+                    for (int u = 0; u < KB; ++u) {
+                        r[u] = kb + u < k1 ? d.req[en[u] >> 1] : zero;
+                        fo[u] = kb + u < k1 ? d.sur_factor[d.nl2_oth[kb + u]] : zero;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < KB; ++u) {
@@ -373,21 +462,21 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
             vol = vmax(vol + delta, zero);
         }
         // outfall (v1_parity.py:361-367)
-        const real hd = node_head(d, n, vol) - d.o_tw[n];
+        const real hd = head_of(inv, As, vol) - o_tw;
         const real sgn = hd > zero ? real(1) : (hd < zero ? real(-1) : zero);
-        real oq = d.o_coef[n] * d.o_area[n] * sqrt(p.two_g * fabs(hd)) * sgn;
-        oq = vmin(oq, vmax(d.o_cap[n], zero));
+        real oq = o_coef * o_area * sqrt(p.two_g * fabs(hd)) * sgn;
+        oq = vmin(oq, vmax(o_cap, zero));
         const real outf = vmin(vol, vmax(oq, zero) * dt);
         vol = vmax(vol - outf, zero);
         // overflow (v1_parity.py:371-404)
-        const real capacity = d.As[n] * vmax(d.dmax[n], zero);
+        const real capacity = As * vmax(dmax, zero);
         const real over = vmax(vol - capacity, zero);
         real returned = zero;
         real wsum = zero;
         if (d.S > 0) {
-            for (int k = e0; k < e1; ++k) wsum = wsum + vmax(d.ne_sw[k], zero);
+            for (int k = e0; k < e1; ++k) wsum = wsum + vmax(k == e0 ? sw0 : d.ne_sw[k], zero);
             for (int k = e0; k < e1; ++k) {
-                const real raw = vmax(d.ne_sw[k], zero);
+                const real raw = vmax(k == e0 ? sw0 : d.ne_sw[k], zero);
                 const real norm = d.snorm[mem] ? raw : raw / vmax(wsum, real(1.0e-12));
                 returned = returned + over * norm;
             }
@@ -395,7 +484,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         }
         vol = vol - returned;
         const real residual = vmax(vol - capacity, zero);
-        const real fb = (!d.explicit_[n] && d.nsc[n] >= 0) ? residual : zero;
+        const real fb = (!expl && nsc >= 0) ? residual : zero;
         vol = vol - fb;
         d.V[n] = vol;
         d.overflow[n] = over;
@@ -408,7 +497,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
     __shared__ double lds[48];
     double v[3] = {out_sum, total_sum, unret_sum};
     block_sum<3>(v, lds);
-    accumulate_slot<3>(part, b, v);
+    acc3.commit(part, b, v);
 }
 
 // Return pass (one thread per receiving cell): weighted entry returns, node
@@ -418,33 +507,45 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
         typename V4<real>::type* __restrict__ state, const real* __restrict__ area, double* __restrict__ part) {
     const int b = blockIdx.x;
     const int j = b * blockDim.x + threadIdx.x;
+    const int mem = d.seg_ret.blk_member[b];
+    const real zero = real(0);
     const int c = d.rc_cells[j];
     const int ke0 = d.rc_eptr[j], ke1 = d.rc_eptr[j + 1];
-    const int mem = d.seg_ret.blk_member[b];
+    const int kf0 = d.rc_fptr[j], kf1 = d.rc_fptr[j + 1];
+    const real A = area[c];
+    const real h0 = state[c].x;
+    // first entry and first fallback node, loaded ahead
+    const int ne0 = d.rc_enode[ahead(ke0, ke1)];
+    const real esw0 = d.rc_esw[ahead(ke0, ke1)];
+    const real over0 = d.overflow[ne0], wsum0 = d.wsum[ne0];
+    const real fb0 = d.fallback[d.rc_fidx[ahead(kf0, kf1)]];
+    SlotAcc<2> acc2;
+    acc2.prefetch(part, b);
+    FH_ISSUE_LOADS_OF("s"(state), "s"(area), "s"(part));
     Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
-    const real zero = real(0);
     double cap_sum = 0.0, loss_sum = 0.0;
     if (j < d.seg_ret.mem_end[mem]) {
         const real dt = (real)ctl->dt_real;
-        const real A = area[c];
-        real h = state[c].x;
+        real h = h0;
         if (d.S > 0) {
             real acc = zero;
             for (int k = ke0; k < ke1; ++k) {
-                const int n = d.rc_enode[k];
-                const real raw = vmax(d.rc_esw[k], zero);
-                const real norm = d.snorm[mem] ? raw : raw / vmax(d.wsum[n], real(1.0e-12));
-                acc = acc + (d.overflow[n] * norm) / A;
+                real raw, ov, ws;
+                if (k == ke0) {
+                    raw = vmax(esw0, zero); ov = over0; ws = wsum0;
+                } else {
+                    const int n = d.rc_enode[k];
+                    raw = vmax(d.rc_esw[k], zero); ov = d.overflow[n]; ws = d.wsum[n];
+                }
+                const real norm = d.snorm[mem] ? raw : raw / vmax(ws, real(1.0e-12));
+                acc = acc + (ov * norm) / A;
             }
             h = h + acc;
         }
         {
             real acc = zero;
-            for (int k = d.rc_fptr[j]; k < d.rc_fptr[j + 1]; ++k) {
-                const int n = d.rc_fidx[k];
-                acc = acc + d.fallback[n] / A;
-            }
+            for (int k = kf0; k < kf1; ++k) acc = acc + (k == kf0 ? fb0 : d.fallback[d.rc_fidx[k]]) / A;
             h = h + acc;
         }
         if (d.R > 0 && d.rc_rptr[j + 1] > d.rc_rptr[j]) {
@@ -469,7 +570,7 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
     __shared__ double lds[32];
     double v[2] = {cap_sum, loss_sum};
     block_sum<2>(v, lds);
-    accumulate_slot<2>(part, b, v);
+    acc2.commit(part, b, v);
 }
 
 }  // namespace fh

@@ -244,6 +244,24 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
     typedef typename V4<real>::type V;
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
     const int m = es.seg.blk_member[b];
+    // every load ahead of the control block: the grid covers the padded edge
+    // space exactly, padding edges point at cell 0 as copy ghosts
+    int lft[EPT], rgt[EPT];
+    real nxs[EPT], nys[EPT], lens[EPT];
+    V sls[EPT], srs[EPT];
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        const int e = (b * EPT + k) * (int)blockDim.x + threadIdx.x;
+        lft[k] = el[e];
+        rgt[k] = er[e];
+        nxs[k] = enx[e]; nys[k] = eny[e]; lens[k] = elen[e];
+    }
+#pragma unroll
+    for (int k = 0; k < EPT; ++k) {
+        sls[k] = state[lft[k]];
+        srs[k] = state[rgt[k] >= 0 ? rgt[k] : lft[k]];   // ghost edges: unused
+    }
+    FH_ISSUE_LOADS_OF("s"(el), "s"(er), "s"(enx), "s"(eny), "s"(elen), "s"(state));
     Ctl* ctl = ctls + m;
     if (!ctl->active) return;
     const int end = es.seg.mem_end[m];
@@ -253,13 +271,12 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
     for (int k = 0; k < EPT; ++k) {
         const int e = (b * EPT + k) * (int)blockDim.x + threadIdx.x;
         if (e >= end) continue;
-        const int l = el[e];
-        const int r = er[e];
-        const real nx = enx[e], ny = eny[e], len = elen[e];
-        const V sl = state[l];
+        const int r = rgt[k];
+        const real nx = nxs[k], ny = nys[k], len = lens[k];
+        const V sl = sls[k];
         real h_r, hu_r, hv_r, z_r;
         if (r >= 0) {
-            const V sr = state[r];
+            const V sr = srs[k];
             h_r = sr.x; hu_r = sr.y; hv_r = sr.z; z_r = sr.w;
         } else if (r == kGhostReflect) {
             const real nm = sl.y * nx + sl.z * ny;
@@ -316,42 +333,63 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
         const real* __restrict__ hk_, double* __restrict__ part) {
     typedef typename V4<real>::type V;
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    const int c = b * blockDim.x + threadIdx.x;
     const int m = seg.blk_member[b];
+    const real zero = real(0), eps = p.eps;
+    // every load of the cell (the grid covers the padded cell space exactly;
+    // padding and inactive cells carry no table entries) ahead of the control block
+    V s = state[c];
+    const real wt0 = wet[c];
+    const int ent[3] = {ce0[c], ce1[c], ce2[c]};
+    // The three gathers are unconditional (a missing entry reads record 0 and
+    // is ignored below), so they issue back to back with no branch between
+    // them; recR is gathered for left entries too and ignored there.
+    V rr[3];
+    real rR[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int en = ent[k];
+        const int pos = en >= 0 ? (en >> 1) : 0;
+        rr[k] = rec[pos];
+        rR[k] = recR[pos];
+    }
+    const real A = area_[c], n2 = n2_[c], rm = rainm[c], f0 = f0_[c], fc = fc_[c], hk = hk_[c];
+    V cc;
+    if (COMP) cc = comp[c];
+    SlotAcc<2> acc2;
+    acc2.prefetch(part, b);
+    FH_ISSUE_LOADS_OF("s"(state), "s"(wet), "s"(ce0), "s"(ce1), "s"(ce2), "s"(rec), "s"(recR), "s"(area_),
+                      "s"(n2_), "s"(rainm), "s"(f0_), "s"(fc_), "s"(hk_), "s"(comp), "s"(part));
     Ctl* ctl = ctls + m;
     if (!ctl->active) return;
-    const int c = b * blockDim.x + threadIdx.x;
-    const int blk0 = seg.mem_blk0[m], nblk = seg.mem_nblk[m];
+    const int blk0 = seg.mem_blk0[m];
     const real dt = (real)ctl->dt_real;
     const real dt_wet = (real)ctl->dt_wet;
     const real rate = (real)ctl->rain_rate;
-    const real zero = real(0), eps = p.eps;
     double rain_v = 0.0, inf_v = 0.0;
     if (b == blk0 && threadIdx.x == 0) {
         ctl->eval_next = (p.fixed_dt <= 0.0 && ctl->override_dt == 0 && (ctl->step % p.interval) == 0) ? 1 : 0;
     }
     if (c < seg.mem_end[m]) {
-        V s = state[c];
         // deferred wet-time increment of the previous step (h here is its final depth)
-        real wt = wet[c];
+        real wt = wt0;
         wt = wt + (s.x > eps ? dt_wet : zero);
-        const int e0 = ce0[c];
+        const int e0 = ent[0];
         if (e0 == kCellInactive) {
             s.x = zero; s.y = zero; s.z = zero;
             state[c] = s;
             wet[c] = wt;
             if (COMP) { V z; z.x = zero; z.y = zero; z.z = zero; z.w = zero; comp[c] = z; }
         } else {
-            const int ent[3] = {e0, ce1[c], ce2[c]};
             real Lh = zero, Lu = zero, Lv = zero, Rh = zero, Ru = zero, Rv = zero;
             bool anyR = false;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int en = ent[k];
                 if (en < 0) continue;
-                const int pos = en >> 1;
-                const V r = rec[pos];
+                const V r = rr[k];
                 if (en & 1) {
-                    Rh = Rh + r.x; Ru = Ru + r.w; Rv = Rv + recR[pos]; anyR = true;
+                    Rh = Rh + r.x; Ru = Ru + r.w; Rv = Rv + rR[k]; anyR = true;
                 } else {
                     Lh = Lh + (-r.x); Lu = Lu + (-r.y); Lv = Lv + (-r.z);
                 }
@@ -359,11 +397,9 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
             const real dh = anyR ? Lh + Rh : Lh;
             const real dhu = anyR ? Lu + Ru : Lu;
             const real dhv = anyR ? Lv + Rv : Lv;
-            const real A = area_[c];
             real h, hu, hv;
             real ch = zero, chu = zero, chv = zero;
             if (COMP) {
-                const V cc = comp[c];
                 h = two_sum<real>(s.x, dt * dh / A + cc.x, ch);
                 hu = two_sum<real>(s.y, dt * dhu / A + cc.y, chu);
                 hv = two_sum<real>(s.z, dt * dhv / A + cc.z, chv);
@@ -379,16 +415,15 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
                 const real hs = vmax(h, eps);
                 const real u = hu / hs, v = hv / hs;
                 const real speed = sqrt(u * u + v * v);
-                const real damping = real(1) + dt * p.g * n2_[c] * speed / pow(hs, p.pow43);
+                const real damping = real(1) + dt * p.g * n2 * speed / pow(hs, p.pow43);
                 if (w) { hu = hu / damping; hv = hv / damping; if (COMP) { chu = chu / damping; chv = chv / damping; } }
                 else { hu = zero; hv = zero; chu = zero; chv = zero; }
             }
             // rain + Horton infiltration (capacity from the wet time of the previous step)
-            const real rain_depth = vmax(rate, zero) * rainm[c] * dt;
+            const real rain_depth = vmax(rate, zero) * rm * dt;
             real h_before, e1 = zero;
             if (COMP) h_before = two_sum<real>(h, rain_depth, e1); else h_before = h + rain_depth;
-            const real f0 = f0_[c], fc = fc_[c];
-            const real capacity = fc + (f0 - fc) * exp(-hk_[c] * wt);
+            const real capacity = fc + (f0 - fc) * exp(-hk * wt);
             const real infil = vmin(h_before, vmax(capacity, zero) * dt);
             real h2, e2 = zero;
             if (COMP) h2 = two_sum<real>(h_before, -infil, e2); else h2 = h_before - infil;
@@ -417,7 +452,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
     __shared__ double lds[16 * 2];
     double v[2] = {rain_v, inf_v};
     block_sum<2>(v, lds);
-    accumulate_slot<2>(part, b, v);
+    acc2.commit(part, b, v);
 }
 
 // Applies the pending wet-time increment of each member's last executed step.

@@ -58,10 +58,13 @@ struct DevArray {
         n = 0;
     }
     // uploads v (at least one element is always allocated so kernels get a valid pointer)
+    // An empty table still gets one zeroed element, so kernels may load entry 0
+    // of any table ahead of their row tests (and use it as an index).
     void set(const std::vector<T>& v, hipStream_t s) {
         const size_t cnt = std::max<size_t>(v.size(), 1);
         if (cnt != n) alloc(cnt);
         if (!v.empty()) FH_CHECK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+        else FH_CHECK(hipMemsetAsync(p, 0, sizeof(T), s));
     }
     void put(const T* host, size_t offset, size_t count, hipStream_t s) {
         if (count) FH_CHECK(hipMemcpyAsync(p + offset, host, count * sizeof(T), hipMemcpyHostToDevice, s));
