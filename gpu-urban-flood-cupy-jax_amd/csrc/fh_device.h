@@ -57,6 +57,10 @@ struct Ctl {
     double override_rain;
     int32_t nonfinite;     // count of non-finite values found by k_finite
     int32_t pad0;
+    // forcing-table search hints: the table position found at an earlier time of
+    // this clock (reset whenever the clock or the forcing is set); used only when
+    // the table's end times are sorted, which the host checks
+    int32_t rain_hint, ext_hint, rain_sorted, ext_sorted;
     double cfl_min;        // result of the last CFL reduction
     unsigned long long cfl_bits;  // running min of the CFL reduction (bits of a non-negative double)
     double diag[kLedgerTerms];    // this step's ledger increments

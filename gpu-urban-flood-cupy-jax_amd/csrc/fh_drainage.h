@@ -41,6 +41,21 @@ struct DrainDev {
     // inlet cells -> inlets (ascending): node, capture limit, node slot
     const int *ic_cells, *ic_ptr, *ic_node, *ic_slot;
     const real* ic_capq;
+    // Row-aligned copies of each row's first table entries (0 where the row has
+    // none), loaded together with the row itself rather than through its CSR
+    // pointer, which saves one dependent memory round trip per kernel.
+    const int* nf_cell;              // [N] first inlet of the node: cell, surcharge limit
+    const real* nf_surq;
+    const int *icf_node, *icf_slot;  // [inlet cells] first inlet: node, node slot, capture limit,
+    const real *icf_capq, *icf_inv, *icf_As;   //   node invert and storage area
+    const int *lk_a0, *lk_a1, *lk_b0, *lk_b1;  // [L] inlet-slot ranges of the two end nodes,
+    const real *lk_inva, *lk_Asa, *lk_invb, *lk_Asb;   //   their inverts and storage areas
+    const int* nle;                  // [KB][N] first KB entries of nl_ent
+    const int *nl2e, *nl2o;          // [KB][N] first KB entries of nl2_ent / nl2_oth
+    const real* nesw0;               // [N] first entry weight
+    const int* rcf_node;             // [return cells] first entry: node, weight; first fallback node
+    const real* rcf_sw;
+    const int* rcf_fb;
     // links
     const int *from, *to, *geom;
     const real *len, *man, *lcap, *linv, *D, *minor, *rw, *rh;
@@ -84,6 +99,9 @@ __device__ __forceinline__ real orifice_q(const KP<real>& p, real dh) {
 // row has it, else entry 0 (every table holds at least one element); values
 // loaded for a missing entry are never used.
 __device__ __forceinline__ int ahead(int k, int k1) { return k < k1 ? k : 0; }
+
+// entries per row kept in the row-aligned copies of the node->link tables
+constexpr int kRowEntries = 4;
 
 // commit helper used by the last block of each ledger kernel
 template <int NT>
@@ -158,8 +176,8 @@ __global__ void __launch_bounds__(256) k_inlet_node_factor(const Ctl* __restrict
     if (k0 == k1) return;
     const real vol = d.V[n];
     const real inv = d.inv[n], As = d.As[n];
-    const real sq0 = d.ni_surq[k0];                 // first inlet, loaded ahead
-    const V s0 = state[d.ni_cell[k0]];
+    const real sq0 = d.nf_surq[n];                  // first inlet, loaded ahead
+    const V s0 = state[d.nf_cell[n]];
     FH_ISSUE_LOADS_OF("s"(state));
     const Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
@@ -199,11 +217,10 @@ __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<r
     const int k0 = d.ic_ptr[j], k1 = d.ic_ptr[j + 1];
     V s = state[c];
     const real A = area[c];
-    // first inlet of the cell, loaded ahead (padding rows point at slot 0 data, never used)
-    const int kf = ahead(k0, k1);
-    const int n0 = d.ic_node[kf], slot0 = d.ic_slot[kf];
-    const real capq0 = d.ic_capq[kf];
-    const real V0 = d.V[n0], inv0 = d.inv[n0], As0 = d.As[n0], sur0 = d.ni_sur[slot0];
+    // first inlet of the cell, loaded ahead (rows without one read node 0 / slot 0, never used)
+    const int n0 = d.icf_node[j], slot0 = d.icf_slot[j];
+    const real capq0 = d.icf_capq[j], inv0 = d.icf_inv[j], As0 = d.icf_As[j];
+    const real V0 = d.V[n0], sur0 = d.ni_sur[slot0];
     SlotAcc<2> acc2;
     acc2.prefetch(part, b);
     FH_ISSUE_LOADS_OF("s"(state), "s"(area), "s"(part));
@@ -254,10 +271,11 @@ __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<r
 
 // Node volume after the inlet exchange (v1_parity.py:292-298):
 // max(V + sum over the node's inlets of (capture - surcharge), 0), for every node.
+// net0 is ni_net[k0], loaded ahead by the caller (unused when the node has no inlet).
 template <typename real>
-__device__ __forceinline__ real post_inlet_volume(const DrainDev<real>& d, int k0, int k1, real vol) {
+__device__ __forceinline__ real post_inlet_volume(const DrainDev<real>& d, int k0, int k1, real vol, real net0) {
     real acc = real(0);
-    for (int k = k0; k < k1; ++k) acc = acc + d.ni_net[k];
+    for (int k = k0; k < k1; ++k) acc = acc + (k == k0 ? net0 : d.ni_net[k]);
     return vmax(vol + acc, real(0));
 }
 
@@ -332,16 +350,17 @@ __global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<r
     const int mem = d.seg_link.blk_member[blockIdx.x];
     const int a = d.from[l], b = d.to[l];
     const LinkData<real> lk = load_link(d, l);
-    const int a0 = d.ni_ptr[a], a1 = d.ni_ptr[a + 1], b0 = d.ni_ptr[b], b1 = d.ni_ptr[b + 1];
+    const int a0 = d.lk_a0[l], a1 = d.lk_a1[l], b0 = d.lk_b0[l], b1 = d.lk_b1[l];
+    const real inva = d.lk_inva[l], Asa = d.lk_Asa[l], invb = d.lk_invb[l], Asb = d.lk_Asb[l];
     const real Va = d.V[a], Vb = d.V[b];
-    const real inva = d.inv[a], Asa = d.As[a], invb = d.inv[b], Asb = d.As[b];
+    const real neta = d.ni_net[ahead(a0, a1)], netb = d.ni_net[ahead(b0, b1)];
     FH_ISSUE_LOADS();
     const Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
     if (l >= d.seg_link.mem_end[mem]) return;
     const real dt = (real)ctl->dt_real;
-    const real va = post_inlet_volume(d, a0, a1, Va);
-    const real vb = a == b ? va : post_inlet_volume(d, b0, b1, Vb);
+    const real va = post_inlet_volume(d, a0, a1, Va, neta);
+    const real vb = a == b ? va : post_inlet_volume(d, b0, b1, Vb, netb);
     d.req[l] = link_request(p, lk, head_of(inva, Asa, va), head_of(invb, Asb, vb), dt);
 }
 
@@ -356,23 +375,19 @@ __global__ void __launch_bounds__(256) k_node_factor(const Ctl* __restrict__ ctl
     const int k0 = d.nl_ptr[n], k1 = d.nl_ptr[n + 1];
     const real vol = d.V[n];
     // the first inlet slot and the first KB link entries, loaded ahead
-    constexpr int KB = 4;
+    constexpr int KB = kRowEntries;
+    const int N = d.N;
     const real net0 = d.ni_net[ahead(i0, i1)];
     int en[KB];
     real r[KB];
 #pragma unroll
-    for (int u = 0; u < KB; ++u) en[u] = d.nl_ent[ahead(k0 + u, k1)];
+    for (int u = 0; u < KB; ++u) en[u] = d.nle[u * N + n];
 #pragma unroll
     for (int u = 0; u < KB; ++u) r[u] = d.req[en[u] >> 1];
     FH_ISSUE_LOADS();
     if (!ctls[mem].active) return;
     if (n >= d.seg_node.mem_end[mem]) return;
-    real vn;
-    {
-        real acc = real(0);
-        for (int k = i0; k < i1; ++k) acc = acc + (k == i0 ? net0 : d.ni_net[k]);
-        vn = vmax(vol + acc, real(0));
-    }
+    const real vn = post_inlet_volume(d, i0, i1, vol, net0);
     d.Vp[n] = vn;
     real total = real(0);
     for (int kb = k0; kb < k1; kb += KB) {
@@ -411,17 +426,21 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
     const real inv = d.inv[n], As = d.As[n], dmax = d.dmax[n];
     const int expl = d.explicit_[n], nsc = d.nsc[n];
     // the first KB link entries and the first entry weight, loaded ahead
-    constexpr int KB = 4;
-    int en[KB];
+    constexpr int KB = kRowEntries;
+    const int N = d.N;
+    int en[KB], ot[KB];
     real r[KB], fo[KB];
 #pragma unroll
-    for (int u = 0; u < KB; ++u) en[u] = d.nl2_ent[ahead(k0 + u, k1)];
+    for (int u = 0; u < KB; ++u) {
+        en[u] = d.nl2e[u * N + n];
+        ot[u] = d.nl2o[u * N + n];
+    }
 #pragma unroll
     for (int u = 0; u < KB; ++u) {
         r[u] = d.req[en[u] >> 1];
-        fo[u] = d.sur_factor[d.nl2_oth[ahead(k0 + u, k1)]];
+        fo[u] = d.sur_factor[ot[u]];
     }
-    const real sw0 = d.ne_sw[ahead(e0, e1)];
+    const real sw0 = d.nesw0[n];
     SlotAcc<3> acc3;
     acc3.prefetch(part, b);
     FH_ISSUE_LOADS_OF("s"(part));
@@ -515,10 +534,10 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
     const real A = area[c];
     const real h0 = state[c].x;
     // first entry and first fallback node, loaded ahead
-    const int ne0 = d.rc_enode[ahead(ke0, ke1)];
-    const real esw0 = d.rc_esw[ahead(ke0, ke1)];
+    const int ne0 = d.rcf_node[j];
+    const real esw0 = d.rcf_sw[j];
     const real over0 = d.overflow[ne0], wsum0 = d.wsum[ne0];
-    const real fb0 = d.fallback[d.rc_fidx[ahead(kf0, kf1)]];
+    const real fb0 = d.fallback[d.rcf_fb[j]];
     SlotAcc<2> acc2;
     acc2.prefetch(part, b);
     FH_ISSUE_LOADS_OF("s"(state), "s"(area), "s"(part));

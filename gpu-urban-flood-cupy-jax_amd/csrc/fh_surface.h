@@ -189,8 +189,17 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
         const int n_rain = bt.rain_n[m];
         const float* rs = rain_start + bt.rain_off[m];
         const float* re = rain_end + bt.rain_off[m];
-        int lo = 0, hi = n_rain;       // searchsorted(end, t, side="right")
-        while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)re[mid] <= t) lo = mid + 1; else hi = mid; }
+        int lo = 0;                    // searchsorted(end, t, side="right")
+        if (ctl->rain_sorted) {
+            // sorted end times: the count of ends <= t only grows with t, so a
+            // forward scan from the position found at an earlier time is exact
+            lo = ctl->rain_hint < n_rain ? ctl->rain_hint : n_rain;
+            while (lo < n_rain && (double)re[lo] <= t) ++lo;
+            ctl->rain_hint = lo;
+        } else {
+            int hi = n_rain;
+            while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)re[mid] <= t) lo = mid + 1; else hi = mid; }
+        }
         const int safe = lo < n_rain ? lo : (n_rain > 0 ? n_rain - 1 : 0);
         const bool valid = (lo < n_rain) && (t >= (double)rs[safe]);
         rate = valid ? (double)rain_rate[bt.rain_off[m] + safe] : 0.0;
@@ -204,8 +213,15 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
     if (n_ext > 0) {
         const float* et = ext_times + bt.ext_off[m];
         const double te = t + 0.5 * (double)dtr;
-        int lo = 0, hi = n_ext;
-        while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)et[mid] <= te) lo = mid + 1; else hi = mid; }
+        int lo = 0;
+        if (ctl->ext_sorted) {         // te = t + dt/2 never decreases along a clock
+            lo = ctl->ext_hint < n_ext ? ctl->ext_hint : n_ext;
+            while (lo < n_ext && (double)et[lo] <= te) ++lo;
+            ctl->ext_hint = lo;
+        } else {
+            int hi = n_ext;
+            while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)et[mid] <= te) lo = mid + 1; else hi = mid; }
+        }
         const int upper = lo;
         int lower = upper - 1; lower = lower < 0 ? 0 : (lower > n_ext - 1 ? n_ext - 1 : lower);
         const int us = upper > n_ext - 1 ? n_ext - 1 : upper;

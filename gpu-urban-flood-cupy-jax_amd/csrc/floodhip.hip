@@ -553,6 +553,14 @@ public:
         s.ext_t.assign(et ? et : nullptr, et ? et + s.T : nullptr);
         s.ext_q.assign(q ? q : nullptr, q ? q + s.T * s.Rcv : nullptr);
         s.rcv = ints(rcv, nrcv);
+        // forcing-table hints start over; the forward scan is used only on sorted tables
+        pull_ctl();
+        fh::Ctl& c = ctl_h[cur];
+        c.rain_hint = 0;
+        c.ext_hint = 0;
+        c.rain_sorted = std::is_sorted(s.rain_e.begin(), s.rain_e.end()) ? 1 : 0;
+        c.ext_sorted = std::is_sorted(s.ext_t.begin(), s.ext_t.end()) ? 1 : 0;
+        push_ctl();
     }
 
     void set_state(const float* h, const float* hu, const float* hv, const float* wet, const float* vol,
@@ -653,6 +661,8 @@ public:
         c.t = t;
         c.cached_cfl = (double)(real)cached;
         c.step = step;
+        c.rain_hint = 0;
+        c.ext_hint = 0;
         c.eval_next = (P.fixed_dt <= 0.0 && (step % kp.interval) == 0) ? 1 : 0;
         push_ctl();
     }
@@ -1049,6 +1059,54 @@ public:
                 rc_enode[k] = snode[rc_eidx[k]];
                 rc_esw[k] = sw[rc_eidx[k]];
             }
+            // row-aligned copies of each row's first entries
+            {
+                const int KB = fh::kRowEntries;
+                std::vector<int> nf_cell(NT, 0), nle((size_t)KB * NT, 0), nl2e((size_t)KB * NT, 0),
+                    nl2o((size_t)KB * NT, 0);
+                std::vector<real> nf_surq(NT, 0), nesw0(NT, 0);
+                for (int n = 0; n < NT; ++n) {
+                    if (ni_ptr[n] < ni_ptr[n + 1]) { nf_cell[n] = ni_cell[ni_ptr[n]]; nf_surq[n] = ni_surq[ni_ptr[n]]; }
+                    for (int u = 0; u < KB; ++u) {
+                        if (nl_ptr[n] + u < nl_ptr[n + 1]) nle[(size_t)u * NT + n] = nl_ent[nl_ptr[n] + u];
+                        if (nl2_ptr[n] + u < nl2_ptr[n + 1]) {
+                            nl2e[(size_t)u * NT + n] = nl2_ent[nl2_ptr[n] + u];
+                            nl2o[(size_t)u * NT + n] = nl2_oth[nl2_ptr[n] + u];
+                        }
+                    }
+                    if (ne_ptr[n] < ne_ptr[n + 1]) nesw0[n] = ne_sw[ne_ptr[n]];
+                }
+                const int CI = icl.total;
+                std::vector<int> icf_node(CI, 0), icf_slot(CI, 0);
+                std::vector<real> icf_capq(CI, 0), icf_inv(CI, 0), icf_As(CI, 1);
+                for (int j = 0; j < CI; ++j) {
+                    if (ic_ptr[j] == ic_ptr[j + 1]) continue;
+                    const int k = ic_ptr[j];
+                    icf_node[j] = ic_node[k]; icf_slot[j] = ic_slot[k]; icf_capq[j] = ic_capq[k];
+                    icf_inv[j] = inv[ic_node[k]]; icf_As[j] = As[ic_node[k]];
+                }
+                std::vector<int> lk_a0(LT, 0), lk_a1(LT, 0), lk_b0(LT, 0), lk_b1(LT, 0);
+                std::vector<real> lk_inva(LT, 0), lk_Asa(LT, 1), lk_invb(LT, 0), lk_Asb(LT, 1);
+                for (int l = 0; l < LT; ++l) {
+                    const int a = from[l], b = to[l];
+                    lk_a0[l] = ni_ptr[a]; lk_a1[l] = ni_ptr[a + 1]; lk_b0[l] = ni_ptr[b]; lk_b1[l] = ni_ptr[b + 1];
+                    lk_inva[l] = inv[a]; lk_Asa[l] = As[a]; lk_invb[l] = inv[b]; lk_Asb[l] = As[b];
+                }
+                const int CR = rcl.total;
+                std::vector<int> rcf_node(CR, 0), rcf_fb(CR, 0);
+                std::vector<real> rcf_sw(CR, 0);
+                for (int j = 0; j < CR; ++j) {
+                    if (rc_eptr[j] < rc_eptr[j + 1]) { rcf_node[j] = rc_enode[rc_eptr[j]]; rcf_sw[j] = rc_esw[rc_eptr[j]]; }
+                    if (rc_fptr[j] < rc_fptr[j + 1]) rcf_fb[j] = rc_fidx[rc_fptr[j]];
+                }
+                d_nf_cell.set(nf_cell, s); d_nf_surq.set(nf_surq, s); d_nle.set(nle, s); d_nl2e.set(nl2e, s);
+                d_nl2o.set(nl2o, s); d_nesw0.set(nesw0, s);
+                d_icf_node.set(icf_node, s); d_icf_slot.set(icf_slot, s); d_icf_capq.set(icf_capq, s);
+                d_icf_inv.set(icf_inv, s); d_icf_As.set(icf_As, s);
+                d_lk_a0.set(lk_a0, s); d_lk_a1.set(lk_a1, s); d_lk_b0.set(lk_b0, s); d_lk_b1.set(lk_b1, s);
+                d_lk_inva.set(lk_inva, s); d_lk_Asa.set(lk_Asa, s); d_lk_invb.set(lk_invb, s); d_lk_Asb.set(lk_Asb, s);
+                d_rcf_node.set(rcf_node, s); d_rcf_sw.set(rcf_sw, s); d_rcf_fb.set(rcf_fb, s);
+            }
             d_ni_cell.set(ni_cell, s); d_ni_surq.set(ni_surq, s);
             std::vector<real> zS((size_t)nslots, 0);
             d_ni_sur.set(zS, s); d_ni_net.set(zS, s);
@@ -1096,6 +1154,13 @@ public:
         d.ne_ptr = d_ne_ptr.p; d.ne_sw = d_ne_sw.p;
         d.ic_cells = d_ic_cells.p; d.ic_ptr = d_ic_ptr.p; d.ic_node = d_ic_node.p; d.ic_slot = d_ic_slot.p;
         d.ic_capq = d_ic_capq.p;
+        d.nf_cell = d_nf_cell.p; d.nf_surq = d_nf_surq.p; d.nle = d_nle.p; d.nl2e = d_nl2e.p; d.nl2o = d_nl2o.p;
+        d.nesw0 = d_nesw0.p;
+        d.icf_node = d_icf_node.p; d.icf_slot = d_icf_slot.p; d.icf_capq = d_icf_capq.p; d.icf_inv = d_icf_inv.p;
+        d.icf_As = d_icf_As.p;
+        d.lk_a0 = d_lk_a0.p; d.lk_a1 = d_lk_a1.p; d.lk_b0 = d_lk_b0.p; d.lk_b1 = d_lk_b1.p;
+        d.lk_inva = d_lk_inva.p; d.lk_Asa = d_lk_Asa.p; d.lk_invb = d_lk_invb.p; d.lk_Asb = d_lk_Asb.p;
+        d.rcf_node = d_rcf_node.p; d.rcf_sw = d_rcf_sw.p; d.rcf_fb = d_rcf_fb.p;
         d.from = d_from.p; d.to = d_to.p; d.geom = d_geom.p; d.len = d_llen.p; d.man = d_man.p; d.lcap = d_lcap.p;
         d.linv = d_linv.p; d.D = d_D.p; d.minor = d_minor.p; d.rw = d_rw.p; d.rh = d_rh.p; d.Q = d_Q.p;
         d.req = d_req.p; d.snode = d_snode.p; d.sw = d_sw.p; d.snorm = d_snorm.p;
@@ -1425,6 +1490,10 @@ public:
     DevArray<real> d_sw;
     DevArray<int> d_rc_cells, d_rc_eptr, d_rc_enode, d_rc_fptr, d_rc_fidx, d_rc_rptr, d_rc_ridx;
     DevArray<real> d_rc_esw;
+    DevArray<int> d_nf_cell, d_nle, d_nl2e, d_nl2o, d_icf_node, d_icf_slot, d_lk_a0, d_lk_a1, d_lk_b0, d_lk_b1,
+        d_rcf_node, d_rcf_fb;
+    DevArray<real> d_nf_surq, d_nesw0, d_icf_capq, d_icf_inv, d_icf_As, d_lk_inva, d_lk_Asa, d_lk_invb, d_lk_Asb,
+        d_rcf_sw;
     DevArray<real> d_rS, d_rcc, d_rrc, d_rstop, d_rstore;
     DevArray<int> d_rcv_qbase, d_rcv_stride, d_xc_cells, d_xc_ptr, d_xc_idx;
     fh::DrainDev<real> dd{};
