@@ -190,21 +190,22 @@ struct Member {
     // state (host copy; authoritative only while the batch is not packed)
     std::vector<real> h, hu, hv, wet, V, Q, rstore;
     fh::Ctl ctl{};
-    // device layout: cpos[c] = position of the caller's cell c, epos[i] = position
-    // of staged edge i (both member-local; empty = as given)
-    std::vector<int> cpos, epos;
+    // device layout (member-local; empty = as given): cpos[c] = position of the
+    // caller's cell c, epos[i] = position of staged edge i, npos / lpos = positions
+    // of the caller's nodes / links.  bfs caches the breadth-first cell order.
+    std::vector<int> bfs, cpos, epos, npos, lpos;
     int cp(int c) const { return cpos.empty() ? c : cpos[(size_t)c]; }
+    int np(int n) const { return npos.empty() ? n : npos[(size_t)n]; }
+    int lp(int l) const { return lpos.empty() ? l : lpos[(size_t)l]; }
 };
 
 // Breadth-first (Cuthill-McKee) order of the cells over interior edges: each
 // component starts at a pseudo-peripheral cell, neighbours are visited by
 // increasing degree.  Cells that share an edge end up a front width apart, so
 // the gathers of the flux and update stages stay within the L2 working set.
-// Edges follow their lower-numbered cell (interior block and ghost block each
-// keep their place), ties in staged order.
 template <typename real>
 void locality_order(Member<real>& m) {
-    const int64_t C = m.C, Ei = m.E_int, E = m.E;
+    const int64_t C = m.C, Ei = m.E_int;
     std::vector<int> deg((size_t)C + 1, 0);
     for (int64_t e = 0; e < Ei; ++e) { deg[(size_t)m.el[e] + 1]++; deg[(size_t)m.er[e] + 1]++; }
     for (int64_t c = 0; c < C; ++c) deg[(size_t)c + 1] += deg[(size_t)c];
@@ -256,8 +257,21 @@ void locality_order(Member<real>& m) {
         bfs(root, true);
         order.insert(order.end(), queue.begin(), queue.end());
     }
+    m.bfs = std::move(order);
+}
+
+// Device positions: cells in breadth-first order; edges follow their lower
+// cell position (interior block and ghost block each keep their place), ties
+// in staged order.  Nodes follow the position of the first cell they exchange
+// water with (their inlets' cells, else their surface cell; nodes with neither
+// keep their order at the end) and links their lower end node, so the node- and
+// link-side gathers of the drainage kernels walk the arrays in the same order
+// as the cells they serve.
+template <typename real>
+void assign_positions(Member<real>& m) {
+    const int64_t C = m.C, Ei = m.E_int, E = m.E;
     m.cpos.assign((size_t)C, 0);
-    for (int64_t i = 0; i < C; ++i) m.cpos[(size_t)order[(size_t)i]] = (int)i;
+    for (int64_t i = 0; i < C; ++i) m.cpos[(size_t)m.bfs[(size_t)i]] = (int)i;
     // edges: by the lower new cell index (interior), by the new left cell (ghost)
     std::vector<int64_t> idx((size_t)E);
     std::iota(idx.begin(), idx.end(), 0);
@@ -269,6 +283,25 @@ void locality_order(Member<real>& m) {
     std::stable_sort(idx.begin() + Ei, idx.end(), [&](int64_t a, int64_t b) { return key(a) < key(b); });
     m.epos.assign((size_t)E, 0);
     for (int64_t i = 0; i < E; ++i) m.epos[(size_t)idx[(size_t)i]] = (int)i;
+    // nodes and links
+    const int64_t N = m.N, L = m.L;
+    std::vector<int64_t> nkey((size_t)N);
+    for (int64_t n = 0; n < N; ++n) nkey[(size_t)n] = m.nsc[n] >= 0 ? m.cpos[(size_t)m.nsc[n]] : C + n;
+    for (int64_t k = 0; k < m.I; ++k) {
+        const int n = m.inode[k];
+        nkey[(size_t)n] = std::min<int64_t>(nkey[(size_t)n], m.cpos[(size_t)m.icell[k]]);
+    }
+    std::vector<int64_t> nidx((size_t)N);
+    std::iota(nidx.begin(), nidx.end(), 0);
+    std::stable_sort(nidx.begin(), nidx.end(), [&](int64_t a, int64_t b) { return nkey[(size_t)a] < nkey[(size_t)b]; });
+    m.npos.assign((size_t)N, 0);
+    for (int64_t i = 0; i < N; ++i) m.npos[(size_t)nidx[(size_t)i]] = (int)i;
+    std::vector<int64_t> lidx((size_t)L);
+    std::iota(lidx.begin(), lidx.end(), 0);
+    auto lkey = [&](int64_t l) { return std::min(m.npos[(size_t)m.lfrom[l]], m.npos[(size_t)m.lto[l]]); };
+    std::stable_sort(lidx.begin(), lidx.end(), [&](int64_t a, int64_t b) { return lkey(a) < lkey(b); });
+    m.lpos.assign((size_t)L, 0);
+    for (int64_t i = 0; i < L; ++i) m.lpos[(size_t)lidx[(size_t)i]] = (int)i;
 }
 
 class EngineBase {
@@ -399,6 +432,7 @@ public:
             ghost_kind.push_back(kind);
         }
         s.C = C_;
+        s.bfs.clear();
         s.cpos.clear();
         s.epos.clear();
         s.E_int = (int64_t)interior.size();
@@ -591,8 +625,16 @@ public:
                 d_comp.put(z.data(), cb, s.C, stream);
             }
             const int mp = pos_of_member(cur);
-            if (vol && s.N) { auto v = conv<real>(vol, s.N); d_V.put(v.data(), nodes.base[mp], s.N, stream); }
-            if (q && s.L) { auto v = conv<real>(q, s.L); d_Q.put(v.data(), links.base[mp], s.L, stream); }
+            if (vol && s.N) {
+                std::vector<real> v((size_t)s.N);
+                for (int64_t n = 0; n < s.N; ++n) v[(size_t)s.np((int)n)] = (real)vol[n];
+                d_V.put(v.data(), nodes.base[mp], s.N, stream);
+            }
+            if (q && s.L) {
+                std::vector<real> v((size_t)s.L);
+                for (int64_t l = 0; l < s.L; ++l) v[(size_t)s.lp((int)l)] = (real)q[l];
+                d_Q.put(v.data(), links.base[mp], s.L, stream);
+            }
             if (rs && s.R) { auto v = conv<real>(rs, s.R); d_rstore.put(v.data(), rrow_base[mp], s.R, stream); }
             FH_CHECK(hipStreamSynchronize(stream));
             return;
@@ -626,8 +668,8 @@ public:
             if (hv) hv[c] = (float)st[q].z;
             if (wet) wet[c] = (float)w[q];
         }
-        if (vol) for (int64_t n = 0; n < s.N; ++n) vol[n] = (float)v[n];
-        if (q) for (int64_t l = 0; l < s.L; ++l) q[l] = (float)qq[l];
+        if (vol) for (int64_t n = 0; n < s.N; ++n) vol[n] = (float)v[(size_t)s.np((int)n)];
+        if (q) for (int64_t l = 0; l < s.L; ++l) q[l] = (float)qq[(size_t)s.lp((int)l)];
         if (rs) for (int64_t k = 0; k < s.R; ++k) rs[k] = (float)r[k];
     }
 
@@ -699,14 +741,18 @@ public:
             std::vector<real> w((size_t)s.C);
             d_state.get(st.data(), cells.base[mp], s.C, stream);
             d_wet.get(w.data(), cells.base[mp], s.C, stream);
-            if (s.N && (int64_t)s.V.size() == s.N) d_V.get(s.V.data(), nodes.base[mp], s.N, stream);
-            if (s.L && (int64_t)s.Q.size() == s.L) d_Q.get(s.Q.data(), links.base[mp], s.L, stream);
+            std::vector<real> v((size_t)s.N), q((size_t)s.L);
+            const bool hv_ = s.N && (int64_t)s.V.size() == s.N, hq = s.L && (int64_t)s.Q.size() == s.L;
+            if (hv_) d_V.get(v.data(), nodes.base[mp], s.N, stream);
+            if (hq) d_Q.get(q.data(), links.base[mp], s.L, stream);
             if (s.R && (int64_t)s.rstore.size() == s.R) d_rstore.get(s.rstore.data(), rrow_base[mp], s.R, stream);
             FH_CHECK(hipStreamSynchronize(stream));
             for (int64_t c = 0; c < s.C; ++c) {
-                const int q = s.cp((int)c);
-                s.h[c] = st[q].x; s.hu[c] = st[q].y; s.hv[c] = st[q].z; s.wet[c] = w[q];
+                const int p = s.cp((int)c);
+                s.h[c] = st[p].x; s.hu[c] = st[p].y; s.hv[c] = st[p].z; s.wet[c] = w[p];
             }
+            if (hv_) for (int64_t n = 0; n < s.N; ++n) s.V[n] = v[(size_t)s.np((int)n)];
+            if (hq) for (int64_t l = 0; l < s.L; ++l) s.Q[l] = q[(size_t)s.lp((int)l)];
         }
     }
 
@@ -719,8 +765,12 @@ public:
     void pack() {
         drop_graph();
         for (auto& m : mem) {
-            if (P.cell_order == 1 && m.cpos.empty() && m.C > 0) locality_order(m);
-            if (P.cell_order != 1) { m.cpos.clear(); m.epos.clear(); }
+            if (P.cell_order == 1 && m.C > 0) {
+                if (m.bfs.empty()) locality_order(m);
+                assign_positions(m);
+            } else {
+                m.cpos.clear(); m.epos.clear(); m.npos.clear(); m.lpos.clear();
+            }
         }
         // member order in the concatenation
         order.resize(M);
@@ -903,25 +953,27 @@ public:
             const int cb = cells.base[i], nb = nodes.base[i], lb = links.base[i];
             snorm[i] = m.snorm;
             for (int64_t n = 0; n < m.N; ++n) {
-                inv[nb + n] = m.inv[n]; As[nb + n] = m.As[n]; dmax[nb + n] = m.dmax[n];
-                oa[nb + n] = m.oarea[n]; oc[nb + n] = m.ocoef[n]; otw[nb + n] = m.otw[n]; ocap[nb + n] = m.ocap[n];
-                V[nb + n] = m.V[n];
-                nsc[nb + n] = m.nsc[n] >= 0 ? m.cp(m.nsc[n]) + cb : -1;
+                const int g = nb + m.np((int)n);
+                inv[g] = m.inv[n]; As[g] = m.As[n]; dmax[g] = m.dmax[n];
+                oa[g] = m.oarea[n]; oc[g] = m.ocoef[n]; otw[g] = m.otw[n]; ocap[g] = m.ocap[n];
+                V[g] = m.V[n];
+                nsc[g] = m.nsc[n] >= 0 ? m.cp(m.nsc[n]) + cb : -1;
             }
             for (int64_t k = 0; k < m.I; ++k) {
                 icell[ibase[i] + k] = m.cp(m.icell[k]) + cb;
-                inode[ibase[i] + k] = m.inode[k] + nb;
+                inode[ibase[i] + k] = m.np(m.inode[k]) + nb;
                 capq[ibase[i] + k] = m.capq[k]; surq[ibase[i] + k] = m.surq[k];
             }
             for (int64_t l = 0; l < m.L; ++l) {
-                from[lb + l] = m.lfrom[l] + nb; to[lb + l] = m.lto[l] + nb; geom[lb + l] = m.geom[l];
-                llen[lb + l] = m.llen[l]; man[lb + l] = m.man[l]; lcap[lb + l] = m.lcap[l]; linv[lb + l] = m.linv[l];
-                D[lb + l] = m.D[l]; minor[lb + l] = m.minor[l]; rw[lb + l] = m.rw[l]; rh[lb + l] = m.rh[l];
-                Q[lb + l] = m.Q[l];
+                const int g = lb + m.lp((int)l);
+                from[g] = m.np(m.lfrom[l]) + nb; to[g] = m.np(m.lto[l]) + nb; geom[g] = m.geom[l];
+                llen[g] = m.llen[l]; man[g] = m.man[l]; lcap[g] = m.lcap[l]; linv[g] = m.linv[l];
+                D[g] = m.D[l]; minor[g] = m.minor[l]; rw[g] = m.rw[l]; rh[g] = m.rh[l];
+                Q[g] = m.Q[l];
             }
             for (int64_t k = 0; k < m.S; ++k) {
                 const bool valid = m.snode[k] >= 0 && m.scell[k] >= 0;
-                snode[sbase[i] + k] = valid ? m.snode[k] + nb : -1;
+                snode[sbase[i] + k] = valid ? m.np(m.snode[k]) + nb : -1;
                 sw[sbase[i] + k] = m.sw[k];
             }
             for (int64_t r = 0; r < m.R; ++r) {
@@ -942,15 +994,19 @@ public:
             std::vector<int> cnt(NT + 1, 0);
             for (int i = 0; i < M; ++i) {
                 const Member<real>& m = mem[order[i]];
-                for (int64_t l = 0; l < m.L; ++l) { cnt[from[links.base[i] + l] + 1]++; cnt[to[links.base[i] + l] + 1]++; }
+                for (int64_t l = 0; l < m.L; ++l) {
+                    const int g = links.base[i] + m.lp((int)l);
+                    cnt[from[g] + 1]++;
+                    cnt[to[g] + 1]++;
+                }
             }
             for (int n = 0; n < NT; ++n) cnt[n + 1] += cnt[n];
             nl_ent.assign(cnt[NT], 0);
             std::vector<int> fill(cnt.begin(), cnt.end() - 1);
             for (int i = 0; i < M; ++i) {
                 const Member<real>& m = mem[order[i]];
-                for (int64_t l = 0; l < m.L; ++l) {
-                    const int g = links.base[i] + (int)l;
+                for (int64_t l = 0; l < m.L; ++l) {   // the caller's link order (reference scatter order)
+                    const int g = links.base[i] + m.lp((int)l);
                     nl_ent[fill[from[g]]++] = g << 1;
                     nl_ent[fill[to[g]]++] = (g << 1) | 1;
                 }
@@ -984,7 +1040,9 @@ public:
         }
         // return cells -> entries, fallback nodes, recharge rows; receiver cells -> receivers
         {
-            std::vector<int> erow(St, -1), frow(NT, -1), rrow(Rt, -1), xrow(Xt, -1);
+            std::vector<int> erow(St, -1), rrow(Rt, -1), xrow(Xt, -1);
+            // fallback rows in the caller's node order (the reference's scatter order), with the node
+            std::vector<int> frow, fnode;
             for (int i = 0; i < M; ++i) {
                 const Member<real>& m = mem[order[i]];
                 std::vector<int> slot((size_t)m.C, -1), xslot((size_t)m.C, -1);
@@ -999,14 +1057,16 @@ public:
                 for (int64_t k = 0; k < m.S; ++k)
                     if (m.snode[k] >= 0 && m.scell[k] >= 0) erow[sbase[i] + k] = slot[m.cp(m.scell[k])];
                 for (int64_t n = 0; n < m.N; ++n) {
-                    const int g = nodes.base[i] + (int)n;
-                    if (!expl[g] && m.nsc[n] >= 0) frow[g] = slot[m.cp(m.nsc[n])];
+                    const int g = nodes.base[i] + m.np((int)n);
+                    frow.push_back(!expl[g] && m.nsc[n] >= 0 ? slot[m.cp(m.nsc[n])] : -1);
+                    fnode.push_back(g);
                 }
                 for (int64_t r = 0; r < m.R; ++r) rrow[rbase[i] + r] = slot[m.cp(m.rcell[r])];
                 for (int64_t j = 0; j < m.Rcv; ++j) xrow[xbase[i] + j] = xslot[m.cp(m.rcv[j])];
             }
             build_csr(erow, rcl.total, rc_eptr, rc_eidx);
             build_csr(frow, rcl.total, rc_fptr, rc_fidx);
+            for (auto& k : rc_fidx) k = fnode[(size_t)k];
             build_csr(rrow, rcl.total, rc_rptr, rc_ridx);
             build_csr(xrow, xcl.total, xc_ptr, xc_idx);
         }
