@@ -165,7 +165,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--uneven", action="store_true", help="also run the 48-member uneven ensemble")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the 48-member uneven ensemble")
     ap.add_argument("--copy-probe-mib", type=int, default=256,
                     help="size of the streaming-copy probe (achievable HBM bandwidth); 0 skips it")
     args = ap.parse_args()
@@ -222,7 +222,7 @@ def main():
     batched = None
     if not args.no_batch and args.batch_members > 0:
         batched = batch_leg(inp, args, tri_steps)
-    uneven = batch_uneven_leg(args) if args.uneven else None
+    uneven = None if (args.no_uneven or args.no_batch) else batch_uneven_leg(args)
     copy = None
     if args.copy_probe_mib > 0:
         import ctypes

@@ -56,6 +56,7 @@ struct DrainDev {
     const int* rcf_node;             // [return cells] first entry: node, weight; first fallback node
     const real* rcf_sw;
     const int* rcf_fb;
+    const real *ic_area, *rc_area;   // cell areas of the inlet cells and of the return cells, in list order
     // links
     const int *from, *to, *geom;
     const real *len, *man, *lcap, *linv, *D, *minor, *rw, *rh;
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<r
     const int c = d.ic_cells[j];
     const int k0 = d.ic_ptr[j], k1 = d.ic_ptr[j + 1];
     V s = state[c];
-    const real A = area[c];
+    const real A = d.ic_area[j];
     // first inlet of the cell, loaded ahead (rows without one read node 0 / slot 0, never used)
     const int n0 = d.icf_node[j], slot0 = d.icf_slot[j];
     const real capq0 = d.icf_capq[j], inv0 = d.icf_inv[j], As0 = d.icf_As[j];
@@ -531,7 +532,7 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
     const int c = d.rc_cells[j];
     const int ke0 = d.rc_eptr[j], ke1 = d.rc_eptr[j + 1];
     const int kf0 = d.rc_fptr[j], kf1 = d.rc_fptr[j + 1];
-    const real A = area[c];
+    const real A = d.rc_area[j];
     const real h0 = state[c].x;
     // first entry and first fallback node, loaded ahead
     const int ne0 = d.rcf_node[j];

@@ -899,7 +899,7 @@ public:
         d_part_flux.set(std::vector<double>(2 * (size_t)std::max(edges.blocks, 1), 0.0), s);
         d_part_upd.set(std::vector<double>(2 * (size_t)std::max(cells.blocks, 1), 0.0), s);
         d_part_fin.alloc(1024);
-        pack_drainage(icells, rcells, xcells);
+        pack_drainage(icells, rcells, xcells, area);
         // control blocks: per member, unchanged
         FH_CHECK(hipStreamSynchronize(s));
         packed = true;
@@ -914,7 +914,7 @@ public:
     }
 
     void pack_drainage(const std::vector<std::vector<int>>& icells, const std::vector<std::vector<int>>& rcells,
-                       const std::vector<std::vector<int>>& xcells) {
+                       const std::vector<std::vector<int>>& xcells, const std::vector<real>& area_of_cell) {
         const hipStream_t s = stream;
         const int NT = nodes.total, LT = links.total;
         // global item bases for inlets, entries, recharge rows, receivers (no padding needed)
@@ -1166,6 +1166,10 @@ public:
                 d_lk_a0.set(lk_a0, s); d_lk_a1.set(lk_a1, s); d_lk_b0.set(lk_b0, s); d_lk_b1.set(lk_b1, s);
                 d_lk_inva.set(lk_inva, s); d_lk_Asa.set(lk_Asa, s); d_lk_invb.set(lk_invb, s); d_lk_Asb.set(lk_Asb, s);
                 d_rcf_node.set(rcf_node, s); d_rcf_sw.set(rcf_sw, s); d_rcf_fb.set(rcf_fb, s);
+                std::vector<real> ic_area(CI, 1), rc_area(CR, 1);
+                for (int j = 0; j < CI; ++j) ic_area[j] = area_of_cell[(size_t)ic_cells[j]];
+                for (int j = 0; j < CR; ++j) rc_area[j] = area_of_cell[(size_t)rc_cells[j]];
+                d_ic_area.set(ic_area, s); d_rc_area.set(rc_area, s);
             }
             d_ni_cell.set(ni_cell, s); d_ni_surq.set(ni_surq, s);
             std::vector<real> zS((size_t)nslots, 0);
@@ -1221,6 +1225,7 @@ public:
         d.lk_a0 = d_lk_a0.p; d.lk_a1 = d_lk_a1.p; d.lk_b0 = d_lk_b0.p; d.lk_b1 = d_lk_b1.p;
         d.lk_inva = d_lk_inva.p; d.lk_Asa = d_lk_Asa.p; d.lk_invb = d_lk_invb.p; d.lk_Asb = d_lk_Asb.p;
         d.rcf_node = d_rcf_node.p; d.rcf_sw = d_rcf_sw.p; d.rcf_fb = d_rcf_fb.p;
+        d.ic_area = d_ic_area.p; d.rc_area = d_rc_area.p;
         d.from = d_from.p; d.to = d_to.p; d.geom = d_geom.p; d.len = d_llen.p; d.man = d_man.p; d.lcap = d_lcap.p;
         d.linv = d_linv.p; d.D = d_D.p; d.minor = d_minor.p; d.rw = d_rw.p; d.rh = d_rh.p; d.Q = d_Q.p;
         d.req = d_req.p; d.snode = d_snode.p; d.sw = d_sw.p; d.snorm = d_snorm.p;
@@ -1552,6 +1557,7 @@ public:
     DevArray<real> d_rc_esw;
     DevArray<int> d_nf_cell, d_nle, d_nl2e, d_nl2o, d_icf_node, d_icf_slot, d_lk_a0, d_lk_a1, d_lk_b0, d_lk_b1,
         d_rcf_node, d_rcf_fb;
+    DevArray<real> d_ic_area, d_rc_area;
     DevArray<real> d_nf_surq, d_nesw0, d_icf_capq, d_icf_inv, d_icf_As, d_lk_inva, d_lk_Asa, d_lk_invb, d_lk_Asb,
         d_rcf_sw;
     DevArray<real> d_rS, d_rcc, d_rrc, d_rstop, d_rstore;

@@ -130,3 +130,33 @@ def stoker_depth(x, t, h_l, h_r, x0, g=9.80665):
     h = np.where((xi > um - np.sqrt(g * hm)) & (xi <= s), hm, h)
     h = np.where(xi > s, h_r, h)
     return h
+
+
+def oracle_run(inputs, duration_s, snapshot_interval_s=3600.0, max_steps_per_hour=20_000):
+    """The oracle's whole-event run in the layout of ``run_hip_v1_parity``: snapshots
+    at the same output targets (the first at 1e-6 s), FP64 ledger, the report fields
+    the gates read (parity_runner.py:264-361 driver, v1-compatible residual)."""
+    from floodhip.runner import output_targets
+    s, c, p, r, st, f = to_oracle(inputs)
+    area = np.asarray(inputs.surface.cell_area, np.float64)
+    clock = orc.OracleClock(0.0, 1.0, 0)
+    ledger = orc.zero_ledger()
+    depth, times, nodes = [], [], []
+    for i, req in enumerate(output_targets(duration_s, snapshot_interval_s)):
+        target = np.float32(1.0e-6) if i == 0 else req
+        st, clock, ledger = orc.advance_to_time(st, clock, ledger, float(target), float(duration_s), s, c, p, r, f,
+                                                max_steps=max_steps_per_hour)
+        depth.append(np.asarray(st.h, np.float32))
+        nodes.append(np.asarray(st.node_volume, np.float32))
+        times.append(clock.time_s)
+    depth_a = np.stack(depth)
+    init_s = float(np.sum(np.asarray(inputs.initial_state.h, np.float64) * area))
+    init_n = float(np.sum(np.asarray(inputs.initial_state.node_volume, np.float64)))
+    fin_s = float(np.sum(depth_a[-1].astype(np.float64) * area))
+    fin_n = float(np.sum(nodes[-1].astype(np.float64)))
+    lhs = init_s + init_n + ledger["rainfall_input"] + ledger["external_boundary_inflow"]
+    rhs = (fin_s + fin_n + ledger["infiltration_loss"] + ledger["outfall_discharge"]
+           + ledger["unreturned_overflow"] + ledger["recharge_loss"] - ledger["boundary_net_flux"])
+    report = {"max_depth_m": float(np.max(depth_a)), **{f"{k}_m3": v for k, v in ledger.items()},
+              "mass_residual_ratio": (lhs - rhs) / max(abs(lhs), 1.0), "step_count": clock.global_step}
+    return {"snapshot_times_s": np.asarray(times, np.float32), "depth_snapshots_m": depth_a, "report": report}
