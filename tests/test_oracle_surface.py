@@ -197,3 +197,45 @@ def test_dam_break_matches_stoker():
     # the plateau depth between rarefaction and shock is captured
     mid = (cen[:, 0] > 215) & (cen[:, 0] < 235)
     assert abs(np.median(st.h[mid]) - np.median(exact[mid])) < 0.01
+
+
+def test_velocity_clamp_caps_speed_and_keeps_direction():
+    """The clamp fixes |u| = max_speed where the speed exceeds it (direction kept),
+    leaves slower cells bit-identical and zeroes the momentum of dry cells."""
+    h = np.array([2.0, 1.0, 0.5, 1e-6, 0.3], np.float64)
+    u = np.array([12.0, 3.0, -4.0, 7.0, 0.0])
+    v = np.array([-9.0, 4.0, 3.0, 1.0, 6.0])
+    st = orc.ParityState(h, h * u, h * v, np.zeros(5), np.zeros(0), np.zeros(0), np.zeros(0))
+    out = orc.clamp_velocity(st, maximum_speed=5.0, dtype=np.float64)
+    speed = np.hypot(out.hu, out.hv) / np.maximum(h, 1e-12)
+    assert speed[0] == pytest.approx(5.0, rel=1e-14)                 # 15 m/s capped to 5
+    assert np.arctan2(out.hv[0], out.hu[0]) == pytest.approx(np.arctan2(-9.0, 12.0), rel=1e-14)
+    assert out.hu[1] == st.hu[1] and out.hv[1] == st.hv[1]            # exactly 5 m/s: untouched
+    assert out.hu[2] == st.hu[2] and out.hv[2] == st.hv[2]            # 5 m/s: untouched
+    assert out.hu[3] == 0.0 and out.hv[3] == 0.0                      # below the wet threshold
+    assert speed[4] == pytest.approx(5.0, rel=1e-14)                  # 6 m/s capped
+
+
+def test_stable_timestep_closed_forms():
+    """Still water of depth h: dt = cfl * min(Lc) / sqrt(g h).  A moving cell:
+    dt = cfl * Lc / (|u| + sqrt(g h)); the minimum picks the binding cell."""
+    g = orc.GRAVITY
+    lc = np.array([4.0, 10.0, 2.5], np.float64)
+    topo = types_surface_with_lc(lc)
+    h = np.full(3, 2.0)
+    st = orc.ParityState(h, np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(0), np.zeros(0), np.zeros(0))
+    assert orc.stable_timestep(st, topo, 0.85, np.float64) == pytest.approx(0.85 * 2.5 / np.sqrt(g * 2.0), rel=1e-14)
+    # cell 1 moves at 30 m/s (3-4-5 direction): lc 10 / (30 + c) can bind only if smaller
+    hu = np.array([0.0, 2.0 * 18.0, 0.0]); hv = np.array([0.0, 2.0 * 24.0, 0.0])
+    st = st._replace(hu=hu, hv=hv)
+    expect = 0.85 * min(4.0 / np.sqrt(g * 2.0), 10.0 / (30.0 + np.sqrt(g * 2.0)), 2.5 / np.sqrt(g * 2.0))
+    assert orc.stable_timestep(st, topo, 0.85, np.float64) == pytest.approx(expect, rel=1e-14)
+
+
+def types_surface_with_lc(lc):
+    """A minimal SurfaceTopology carrying only what stable_timestep reads."""
+    n = lc.shape[0]
+    fields = {f: np.zeros(0) for f in orc.SurfaceTopology._fields}
+    fields["characteristic_length"] = lc
+    fields["cell_area"] = np.ones(n)
+    return orc.SurfaceTopology(**fields)
