@@ -339,14 +339,26 @@ __device__ __forceinline__ real two_sum(real a, real b, real& err) {
 // (:245-256), and the pending wet-time increment of the previous step
 // (:461-463).  COMP carries a per-cell TwoSum compensation of h, hu, hv.
 // ---------------------------------------------------------------------------
-template <typename real, bool COMP>
+// Per-cell source parameters of the update: Manning n^2, rain multiplier (with
+// the active mask folded in) and the Horton f0, fc, k.  Either one array each,
+// or -- when the batch has at most 65,535 distinct (n^2, m, f0, fc, k) tuples,
+// as land-cover classes give -- a 16-bit class per cell and a class table
+// {n^2, m, f0, fc} + k, which holds the same values in 2 bytes per cell.
+template <typename real>
+struct CellSources {
+    const real *n2, *rainm, *f0, *fc, *hk;
+    const unsigned short* cls;
+    const typename V4<real>::type* tab;
+    const real* tab_hk;
+};
+
+template <typename real, bool COMP, bool CLS>
 __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real> p, Seg seg,
         const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
         const typename V4<real>::type* __restrict__ rec, const real* __restrict__ recR,
         typename V4<real>::type* __restrict__ state, typename V4<real>::type* __restrict__ comp,
-        real* __restrict__ wet, const real* __restrict__ area_, const real* __restrict__ n2_,
-        const real* __restrict__ rainm, const real* __restrict__ f0_, const real* __restrict__ fc_,
-        const real* __restrict__ hk_, double* __restrict__ part) {
+        real* __restrict__ wet, const real* __restrict__ area_, CellSources<real> src,
+        double* __restrict__ part) {
     typedef typename V4<real>::type V;
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
     const int c = b * blockDim.x + threadIdx.x;
@@ -369,13 +381,22 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
         rr[k] = rec[pos];
         rR[k] = recR[pos];
     }
-    const real A = area_[c], n2 = n2_[c], rm = rainm[c], f0 = f0_[c], fc = fc_[c], hk = hk_[c];
+    const real A = area_[c];
+    real n2, rm, f0, fc, hk;
+    if (CLS) {
+        const int k = src.cls[c];
+        const V t = src.tab[k];
+        n2 = t.x; rm = t.y; f0 = t.z; fc = t.w;
+        hk = src.tab_hk[k];
+    } else {
+        n2 = src.n2[c]; rm = src.rainm[c]; f0 = src.f0[c]; fc = src.fc[c]; hk = src.hk[c];
+    }
     V cc;
     if (COMP) cc = comp[c];
     SlotAcc<2> acc2;
     acc2.prefetch(part, b);
     FH_ISSUE_LOADS_OF("s"(state), "s"(wet), "s"(ce0), "s"(ce1), "s"(ce2), "s"(rec), "s"(recR), "s"(area_),
-                      "s"(n2_), "s"(rainm), "s"(f0_), "s"(fc_), "s"(hk_), "s"(comp), "s"(part));
+                      "s"(comp), "s"(part));
     Ctl* ctl = ctls + m;
     if (!ctl->active) return;
     const int blk0 = seg.mem_blk0[m];

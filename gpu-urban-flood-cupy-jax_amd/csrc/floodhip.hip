@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -884,6 +885,7 @@ public:
         d_ce0.set(ce0, s); d_ce1.set(ce1, s); d_ce2.set(ce2, s);
         d_state.set(st, s); d_wet.set(wet, s); d_area.set(area, s); d_n2.set(n2, s); d_lc.set(lc, s);
         d_rainm.set(rainm, s); d_f0.set(f0, s); d_fc.set(fc, s); d_k.set(hk, s);
+        build_source_classes(n2, rainm, f0, fc, hk);
         if (P.compensated) {
             std::vector<V> z(cells.total);
             for (auto& v : z) { v.x = 0; v.y = 0; v.z = 0; v.w = 0; }
@@ -905,6 +907,40 @@ public:
         packed = true;
         dirty = false;
         any_drainage = nodes.total > 0;
+    }
+
+    // Class table of the per-cell source tuples (n^2, m, f0, fc, k), compared bit
+    // for bit; used by the update when the batch has at most 65,535 of them.
+    void build_source_classes(const std::vector<real>& n2, const std::vector<real>& rm, const std::vector<real>& f0,
+                              const std::vector<real>& fc, const std::vector<real>& hk) {
+        struct Key {
+            real v[5];
+            bool operator<(const Key& o) const { return std::memcmp(v, o.v, sizeof(v)) < 0; }
+        };
+        std::map<Key, int> ids;
+        std::vector<unsigned short> cls(n2.size(), 0);
+        std::vector<V> tab;
+        std::vector<real> tab_hk;
+        use_classes = true;
+        for (size_t c = 0; c < n2.size(); ++c) {
+            Key k{{n2[c], rm[c], f0[c], fc[c], hk[c]}};
+            auto it = ids.find(k);
+            if (it == ids.end()) {
+                if (ids.size() >= 65535) { use_classes = false; break; }
+                it = ids.emplace(k, (int)tab.size()).first;
+                V t;
+                t.x = n2[c]; t.y = rm[c]; t.z = f0[c]; t.w = fc[c];
+                tab.push_back(t);
+                tab_hk.push_back(hk[c]);
+            }
+            cls[c] = (unsigned short)it->second;
+        }
+        if (!P.source_classes) use_classes = false;
+        if (use_classes) {
+            d_cls.set(cls, stream); d_cls_tab.set(tab, stream); d_cls_hk.set(tab_hk, stream);
+        } else {
+            d_cls.release(); d_cls_tab.release(); d_cls_hk.release();
+        }
     }
 
     static std::vector<int64_t> counts_of(const std::vector<std::vector<int>>& v) {
@@ -1272,14 +1308,16 @@ public:
 #undef FH_FLUX
     }
     void launch_update() {
-        if (P.compensated)
-            go(fh::k_update<real, true>, cells.blocks, bs, ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p, d_ce2.p, d_rec.p,
-               d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p,
-               d_part_upd.p);
-        else
-            go(fh::k_update<real, false>, cells.blocks, bs, ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p, d_ce2.p, d_rec.p,
-               d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p,
-               d_part_upd.p);
+        const fh::CellSources<real> src{d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p, d_cls.p, d_cls_tab.p, d_cls_hk.p};
+#define FH_UPDATE(COMP_, CLS_)                                                                              \
+        go(fh::k_update<real, COMP_, CLS_>, cells.blocks, bs, ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p, d_ce2.p, \
+           d_rec.p, d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, src, d_part_upd.p)
+        if (P.compensated) {
+            if (use_classes) FH_UPDATE(true, true); else FH_UPDATE(true, false);
+        } else {
+            if (use_classes) FH_UPDATE(false, true); else FH_UPDATE(false, false);
+        }
+#undef FH_UPDATE
     }
 
     template <class Timed>
@@ -1323,27 +1361,39 @@ public:
         launch_drainage([](const char*, double, auto&& fn) { fn(); });
     }
 
+    // Two captured step graphs: the full chunk and a short one for the last
+    // steps before a target, so an advance ends with few no-op steps.
+    struct StepGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int steps = 0;
+    };
+    StepGraph graphs[2];
+
     void drop_graph() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        gexec = nullptr;
-        graph = nullptr;
-        graph_steps = 0;
+        for (auto& g : graphs) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            if (g.graph) (void)hipGraphDestroy(g.graph);
+            g = StepGraph{};
+        }
     }
 
-    void build_graph(int k) {
-        drop_graph();
+    void build_graph(StepGraph& g, int k) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        if (g.graph) (void)hipGraphDestroy(g.graph);
+        g = StepGraph{};
         FH_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < k; ++i) launch_step();
-        FH_CHECK(hipStreamEndCapture(stream, &graph));
-        FH_CHECK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-        graph_steps = k;
+        FH_CHECK(hipStreamEndCapture(stream, &g.graph));
+        FH_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        g.steps = k;
     }
 
     void launch_chunk(int k) {
         if (P.use_graph) {
-            if (!gexec || graph_steps != k) build_graph(k);
-            FH_CHECK(hipGraphLaunch(gexec, stream));
+            StepGraph& g = graphs[k == std::max(1, P.steps_per_launch) ? 0 : 1];
+            if (!g.exec || g.steps != k) build_graph(g, k);
+            FH_CHECK(hipGraphLaunch(g.exec, stream));
         } else {
             for (int i = 0; i < k; ++i) launch_step();
         }
@@ -1400,9 +1450,19 @@ public:
                 min_rem = std::min(min_rem, (target - ctl_h[m].t) / dt_est);
                 budget_left = std::max<int64_t>(budget_left, max_steps - ctl_h[m].interval_step);
             }
-            int chunks = (int)std::min<double>(8.0, std::max(1.0, std::floor(min_rem / K)));
-            chunks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, (budget_left + K - 1) / K));
-            for (int i = 0; i < chunks; ++i) launch_chunk(K);
+            const int Kt = std::min(K, kTailSteps);
+            const double rem = std::min(min_rem, (double)budget_left);
+            if (rem < K && Kt < K) {
+                // close to the target or the step budget: short chunks, so the advance
+                // ends with at most Kt - 1 no-op steps
+                int n = (int)std::min<double>(8.0, std::ceil(std::max(rem, 1.0) / Kt));
+                n = (int)std::max<int64_t>(1, std::min<int64_t>(n, (budget_left + Kt - 1) / Kt));
+                for (int i = 0; i < n; ++i) launch_chunk(Kt);
+            } else {
+                int chunks = (int)std::min<double>(8.0, std::max(1.0, std::floor(min_rem / K)));
+                chunks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, (budget_left + K - 1) / K));
+                for (int i = 0; i < chunks; ++i) launch_chunk(K);
+            }
             pull_ctl();
         }
         fold();
@@ -1516,6 +1576,7 @@ public:
         b += d_el.bytes() + d_er.bytes() + d_nx.bytes() + d_ny.bytes() + d_len.bytes();
         b += d_rec.bytes() + d_recR.bytes() + d_state.bytes() + d_wet.bytes() + d_comp.bytes();
         b += d_area.bytes() + d_n2.bytes() + d_lc.bytes() + d_rainm.bytes() + d_f0.bytes() + d_fc.bytes() + d_k.bytes();
+        b += d_cls.bytes() + d_cls_tab.bytes() + d_cls_hk.bytes();
         b += d_ce0.bytes() + d_ce1.bytes() + d_ce2.bytes();
         b += d_inv.bytes() * 12 + d_ni_cell.bytes() * 4 + d_nl_ent.bytes() * 3 + d_ic_node.bytes() * 3 +
              d_from.bytes() * 16;
@@ -1540,6 +1601,10 @@ public:
     DevArray<int> d_gblk0, d_gnblk, d_cell0, d_cell1, d_rain_off, d_rain_n, d_ext_off, d_ext_n;
     DevArray<int> d_el, d_er, d_ce0, d_ce1, d_ce2;
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
+    DevArray<unsigned short> d_cls;   // source class per cell (when use_classes)
+    DevArray<V> d_cls_tab;            // class -> {n^2, m, f0, fc}
+    DevArray<real> d_cls_hk;          // class -> k
+    bool use_classes = false;
     DevArray<V> d_rec, d_state, d_comp;
     DevArray<double> d_part_begin, d_part_flux, d_part_upd, d_part_fin, d_part_inlet, d_part_node, d_part_ret,
         d_part_ext;
@@ -1563,9 +1628,7 @@ public:
     DevArray<real> d_rS, d_rcc, d_rrc, d_rstop, d_rstore;
     DevArray<int> d_rcv_qbase, d_rcv_stride, d_xc_cells, d_xc_ptr, d_xc_idx;
     fh::DrainDev<real> dd{};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_steps = 0;
+    static constexpr int kTailSteps = 4;
 };
 
 struct Ctx {
@@ -1629,6 +1692,7 @@ void fh_default_params(fh_params* p) {
     p->edges_per_thread = 1;
     p->pack_order = 0;
     p->cell_order = 1;
+    p->source_classes = 1;
 }
 
 const char* fh_version(void) { return "floodhip 0.2.0 (gfx950)"; }

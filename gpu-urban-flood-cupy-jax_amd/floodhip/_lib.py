@@ -26,7 +26,7 @@ class FhParams(C.Structure):
         ("precision", C.c_int32), ("use_graph", C.c_int32), ("steps_per_launch", C.c_int32),
         ("block_size", C.c_int32), ("xcd_remap", C.c_int32), ("device", C.c_int32),
         ("edges_per_thread", C.c_int32), ("pack_order", C.c_int32),
-        ("cell_order", C.c_int32),
+        ("cell_order", C.c_int32), ("source_classes", C.c_int32),
     ]
 
 

@@ -80,6 +80,7 @@ class SolverOptions(NamedTuple):
     edges_per_thread: int = 1
     pack_order: int = 0
     cell_order: int = 1
+    source_classes: int = 1
     cfl: float = 0.85
     cfl_interval: int = 32
     reuse_factor: float = 0.8
@@ -119,6 +120,7 @@ class HipParitySolver:
         p.edges_per_thread = int(opts.edges_per_thread)
         p.pack_order = int(opts.pack_order)
         p.cell_order = int(opts.cell_order)
+        p.source_classes = int(opts.source_classes)
         self._ctx = self.lib.fh_create_batch(C.byref(p), int(members))
         if not self._ctx:
             msg = self.lib.fh_global_error().decode()

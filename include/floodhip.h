@@ -69,6 +69,9 @@ typedef struct {
     int32_t pack_order;         /* batch member order: 0 as given, 1 largest first, 2 large/small alternating */
     int32_t cell_order;         /* device layout of cells and edges: 0 as given, 1 breadth-first (Cuthill-McKee)
                                    locality order; results are returned in the caller's order either way */
+    int32_t source_classes;     /* 1: store the per-cell source tuples (Manning n^2, rain multiplier, Horton
+                                   f0, fc, k) as a 16-bit class per cell plus a class table when there are
+                                   at most 65,535 distinct tuples; 0: one array each.  Same values. */
 } fh_params;
 
 typedef struct {

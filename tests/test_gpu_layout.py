@@ -51,3 +51,16 @@ def test_state_roundtrip_through_permuted_layout():
     assert np.array_equal(back.h, h)
     assert np.array_equal(back.hu, hu)
     assert np.array_equal(back.wet_time_s, np.arange(C, dtype=np.float32))
+
+
+@pytest.mark.parametrize("compensated", [False, True])
+def test_source_classes_equal_full_arrays(compensated):
+    """The 16-bit source classes hold the same (n^2, m, f0, fc, k) values as the full
+    per-cell arrays: identical states and ledgers."""
+    inp = synth.make_catchment(8000, seed=synth.BASE_SEED + 34, rain_hours=1, rain_mm_h=[100.0])
+    a, ca, la = _run(inp, 1, compensated=compensated, source_classes=0)
+    b, cb, lb = _run(inp, 1, compensated=compensated, source_classes=1)
+    assert ca["global_step"] == cb["global_step"]
+    for name in ("h", "hu", "hv", "wet_time_s", "node_volume", "link_flow", "recharge_storage"):
+        assert np.array_equal(getattr(a, name), getattr(b, name)), name
+    assert np.array_equal(la, lb)
