@@ -61,6 +61,10 @@ struct Ctl {
     // this clock (reset whenever the clock or the forcing is set); used only when
     // the table's end times are sorted, which the host checks
     int32_t rain_hint, ext_hint, rain_sorted, ext_sorted;
+    // the rain interval at rain_hint (sorted tables): its end (the next end time
+    // after t; -inf forces a lookup), start and rate -- a step whose t is still
+    // below that end reads its rate here instead of in the table
+    double rain_next_end, rain_cur_start, rain_cur_rate;
     double cfl_min;        // result of the last CFL reduction
     unsigned long long cfl_bits;  // running min of the CFL reduction (bits of a non-negative double)
     double diag[kLedgerTerms];    // this step's ledger increments

@@ -110,6 +110,7 @@ struct BeginTables {
     const int* rain_n;       // [M] number of rain intervals
     const int* ext_off;      // [M] offset of the member's external time table
     const int* ext_n;        // [M] number of external times (0: no external inflow)
+    int single;              // one member: every block belongs to member 0
 };
 
 // ---------------------------------------------------------------------------
@@ -124,7 +125,7 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
         const float* __restrict__ rain_rate, const float* __restrict__ ext_times) {
     __shared__ double lds[16 * 2];
     const int b = blockIdx.x;
-    const int m = bt.seg.blk_member[b];
+    const int m = bt.single ? 0 : bt.seg.blk_member[b];
     Ctl* ctl = ctls + m;
     const int blk0 = bt.seg.mem_blk0[m], nblk = bt.seg.mem_nblk[m];
     const bool eval = ctl->eval_next != 0;
@@ -192,17 +193,28 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
         int lo = 0;                    // searchsorted(end, t, side="right")
         if (ctl->rain_sorted) {
             // sorted end times: the count of ends <= t only grows with t, so a
-            // forward scan from the position found at an earlier time is exact
-            lo = ctl->rain_hint < n_rain ? ctl->rain_hint : n_rain;
-            while (lo < n_rain && (double)re[lo] <= t) ++lo;
-            ctl->rain_hint = lo;
+            // forward scan from the position found at an earlier time is exact;
+            // while t stays below the cached next end the position is unchanged
+            if (!(t < ctl->rain_next_end)) {
+                lo = ctl->rain_hint < n_rain ? ctl->rain_hint : n_rain;
+                while (lo < n_rain && (double)re[lo] <= t) ++lo;
+                const int safe = lo < n_rain ? lo : (n_rain > 0 ? n_rain - 1 : 0);
+                ctl->rain_hint = lo;
+                ctl->rain_next_end = lo < n_rain ? (double)re[lo] : INFINITY;
+                ctl->rain_cur_start = n_rain > 0 ? (double)rs[safe] : 0.0;
+                ctl->rain_cur_rate = n_rain > 0 ? (double)rain_rate[bt.rain_off[m] + safe] : 0.0;
+            } else {
+                lo = ctl->rain_hint;
+            }
+            const bool valid = (lo < n_rain) && (t >= ctl->rain_cur_start);
+            rate = valid ? ctl->rain_cur_rate : 0.0;
         } else {
             int hi = n_rain;
             while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)re[mid] <= t) lo = mid + 1; else hi = mid; }
+            const int safe = lo < n_rain ? lo : (n_rain > 0 ? n_rain - 1 : 0);
+            const bool valid = (lo < n_rain) && (t >= (double)rs[safe]);
+            rate = valid ? (double)rain_rate[bt.rain_off[m] + safe] : 0.0;
         }
-        const int safe = lo < n_rain ? lo : (n_rain > 0 ? n_rain - 1 : 0);
-        const bool valid = (lo < n_rain) && (t >= (double)rs[safe]);
-        rate = valid ? (double)rain_rate[bt.rain_off[m] + safe] : 0.0;
     }
     const real dtr = (real)dt;
     ctl->dt_real = (double)dtr;

@@ -357,6 +357,7 @@ public:
             const double inf = INFINITY;
             std::memcpy(&m.ctl.cfl_bits, &inf, sizeof(inf));
             m.ctl.eval_next = P.fixed_dt > 0.0 ? 0 : 1;
+            m.ctl.rain_next_end = -INFINITY;
         }
         for (int m = 0; m < M; ++m) ctl_h[m] = mem[m].ctl;
         FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl) * M, hipMemcpyHostToDevice, stream));
@@ -593,6 +594,7 @@ public:
         fh::Ctl& c = ctl_h[cur];
         c.rain_hint = 0;
         c.ext_hint = 0;
+        c.rain_next_end = -INFINITY;
         c.rain_sorted = std::is_sorted(s.rain_e.begin(), s.rain_e.end()) ? 1 : 0;
         c.ext_sorted = std::is_sorted(s.ext_t.begin(), s.ext_t.end()) ? 1 : 0;
         push_ctl();
@@ -706,6 +708,7 @@ public:
         c.step = step;
         c.rain_hint = 0;
         c.ext_hint = 0;
+        c.rain_next_end = -INFINITY;
         c.eval_next = (P.fixed_dt <= 0.0 && (step % kp.interval) == 0) ? 1 : 0;
         push_ctl();
     }
@@ -1276,7 +1279,8 @@ public:
 
     // ------------------------------------------------------------------ stepping
     fh::BeginTables begin_tables() const {
-        return fh::BeginTables{seg_begin.seg(), d_cell0.p, d_cell1.p, d_rain_off.p, d_rain_n.p, d_ext_off.p, d_ext_n.p};
+        return fh::BeginTables{seg_begin.seg(), d_cell0.p, d_cell1.p, d_rain_off.p, d_rain_n.p, d_ext_off.p, d_ext_n.p,
+                               M == 1 ? 1 : 0};
     }
 
     // Stage launches go through here.  While a kernel is being timed the launch
