@@ -100,3 +100,35 @@ def test_coupled_bitwise_repeatable():
             st = s.get_state()
             outs.append(b"".join(np.asarray(x).tobytes() for x in st) + np.asarray(tuple(s.ledger())).tobytes())
     assert outs[0] == outs[1]
+
+
+@pytest.mark.parametrize("make", [
+    lambda: small_network_scenario(seed=21, h_init=0.2, with_external=False),
+    lambda: synth.make_catchment(8000, seed=synth.BASE_SEED + 42, rain_hours=1, rain_mm_h=[150.0]),
+])
+def test_coupled_fp64_kernels_match_fp64_oracle(make):
+    """The coupled step instantiated for double against the FP64 oracle, fixed dt,
+    100 steps: both round every operation in double (FMA contraction off) and sum
+    in the same orders, so depths agree to 1e-10 m; node volumes and link flows
+    cross the boundary as float32 and agree to float32 rounding.  (External inflow
+    is left out: the oracle interpolates the inflow table in float32.)"""
+    inp = make()
+    io = to_oracle(inp, np.float64)
+    st = io[4]
+    dt, n = 0.4, 100
+    for i in range(n):
+        rate = orc.rain_rate(io[5], i * dt)
+        q = orc.external_q(io[5], i * dt + 0.5 * dt)
+        st, _ = oracle_step(io, st, dt, rain=rate, q=q, dtype=np.float64)
+    with _solver(inp, precision=64, fixed_dt=dt) as s:
+        # in double, 100 x 0.4 sums to just below 40: stop on the step count
+        clock, _ = s.advance_to(n * dt - 1e-6, 1e9, max_steps=n)
+        h, hu, hv = s.get_state_f64()
+        got = s.get_state()
+    assert clock["global_step"] == n
+    assert np.max(np.abs(h - st.h)) <= 1e-10
+    assert np.max(np.abs(hu - st.hu)) <= 1e-10
+    v_ref = st.node_volume.astype(np.float32)
+    assert np.max(np.abs(got.node_volume - v_ref) / np.maximum(np.abs(v_ref), 1.0)) <= 2e-7
+    q_ref = st.link_flow.astype(np.float32)
+    assert np.max(np.abs(got.link_flow - q_ref) / np.maximum(np.abs(q_ref), 1e-3)) <= 2e-7
