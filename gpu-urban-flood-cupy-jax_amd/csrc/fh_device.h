@@ -29,44 +29,62 @@ enum LedgerTerm {
 enum Counter { C_BEGIN = 0, C_FLUX, C_UPDATE, C_INLET, C_NODE, C_RETURN, C_EXTERNAL, C_FINITE,
                C_COUNT = 16 };
 
-// Step control block, resident in device memory.  Written only by the last
-// block of k_begin (clock, dt, forcing lookups, flags), by block 0 of
-// k_update (eval_next) and by the last block of each ledger-committing kernel
-// (diag/ledger terms); every other access is a read.
+// Start-of-step control state of one member, double-buffered by step parity:
+// every block of a step's update kernel reads clk[p] and the update's first
+// block writes clk[1 - p] for the next step, so no block ever reads a clock
+// that another block of the same launch is writing.
+// The rain interval of a time t: the searchsorted(end, t, side="right")
+// position `hint` of parity_runner.py:96-102, the start and rate of that
+// interval (start +inf when past the last one) and its end, the time up to which
+// the same position holds (sorted end times; -inf: look the position up).
+struct RainCache {
+    double next_end, start, rate;
+    int32_t hint, pad;
+};
+
+struct Clock {
+    double t;              // FP64 simulated time at the start of the step
+    double cached_cfl;     // last evaluated CFL dt (state precision value)
+    double dt_prev;        // dt of the previous executed step (its pending wet-time increment)
+    RainCache rain;        // rain interval found for t
+    int64_t step;          // global step counter (steps executed)
+    int64_t interval_step; // steps executed in the current advance call
+    int64_t cfl_evals;
+    int32_t eval;          // this step evaluates the CFL condition
+    int32_t active;        // this step executes: t < target && interval_step < max_steps
+    int32_t pending_wet;   // the previous executed step's wet-time increment is pending
+    // inflow-table search hint: the table position found at an earlier time of
+    // this clock (reset whenever the clock or the forcing is set); used only when
+    // the table's times are sorted, which the host checks
+    int32_t ext_hint;
+};
+
+// Step control block of one member, resident in device memory.  The clock of
+// the next step is written only by the first block of the update kernel; the
+// outputs of the current step (dt, rain rate, inflow interpolation, active) are
+// written there too, for the drainage kernels and the host; ledger terms are
+// written only by k_fold.
 struct Ctl {
-    double t;              // FP64 simulated time (start of the next step)
-    double t_step;         // start time of the current step
+    Clock clk[2];
+    unsigned long long cfl_bits[2];   // running CFL minimum of the step of that parity
     double target;         // stop when t >= target
     double final_t;        // event end time (dt cap final_t - t)
-    double dt;             // dt of the current step, as applied (== (double)dt_real)
-    double dt_real;        // dt of the current step in state precision (stored as double)
-    double dt_wet;         // dt of the previous executed step: pending wet-time increment
-    double cached_cfl;     // last evaluated CFL dt (state precision value)
-    double rain_rate;      // rain rate of the current step (state precision value)
-    double ext_frac;       // external inflow interpolation fraction at t + dt/2
-    int64_t step;          // global step counter
-    int64_t interval_step; // steps taken in the current advance call
     int64_t max_steps;     // step budget of the current advance call
-    int64_t cfl_evals;
-    int32_t ext_lower, ext_upper;
-    int32_t active;        // the current step executes
-    int32_t pending_wet;   // a wet-time increment is pending (applied by the next update / finalize)
-    int32_t eval_next;     // the next step evaluates the CFL condition
-    int32_t override_dt;   // host-driven single step: dt and rain given by the host
     double override_dt_value;
     double override_rain;
+    int32_t override_dt;   // host-driven single step: dt and rain given by the host
+    int32_t rain_sorted, ext_sorted;
+    // outputs of the current step
+    int32_t active;        // the current step executes
+    double dt;             // dt of the current step, as applied (== (double)dt_real)
+    double dt_real;        // dt of the current step in state precision (stored as double)
+    double rain_rate;      // rain rate of the current step (state precision value)
+    double t_step;         // start time of the current step
+    double ext_frac;       // external inflow interpolation fraction at t + dt/2
+    int32_t ext_lower, ext_upper;
     int32_t nonfinite;     // count of non-finite values found by k_finite
     int32_t pad0;
-    // forcing-table search hints: the table position found at an earlier time of
-    // this clock (reset whenever the clock or the forcing is set); used only when
-    // the table's end times are sorted, which the host checks
-    int32_t rain_hint, ext_hint, rain_sorted, ext_sorted;
-    // the rain interval at rain_hint (sorted tables): its end (the next end time
-    // after t; -inf forces a lookup), start and rate -- a step whose t is still
-    // below that end reads its rate here instead of in the table
-    double rain_next_end, rain_cur_start, rain_cur_rate;
     double cfl_min;        // result of the last CFL reduction
-    unsigned long long cfl_bits;  // running min of the CFL reduction (bits of a non-negative double)
     double diag[kLedgerTerms];    // this step's ledger increments
     double ledger[kLedgerTerms];  // cumulative FP64 ledger
     unsigned counters[C_COUNT];
@@ -106,17 +124,80 @@ __device__ __forceinline__ int xcd_block(int i, int G, int remap) {
     return x * q + (x < r ? x : r) + j;
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+// DPP lane moves within 16-lane rows (gfx9 dpp_ctrl encodings).
+enum DppCtrl {
+    kDppQuadXor1 = 0xB1,      // quad_perm [1,0,3,2]
+    kDppQuadXor2 = 0x4E,      // quad_perm [2,3,0,1]
+    kDppRowHalfMirror = 0x141,
+    kDppRowMirror = 0x140,
+};
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum (MIN = false) or minimum over each 16-lane row; every lane of the row
+// ends with the same value (each step combines a lane with its mirror, so both
+// sides add the same two operands).
+template <bool MIN, typename T>
+__device__ __forceinline__ T row_reduce(T v) {
+    auto op = [](T a, T b) { return MIN ? (b < a ? b : a) : a + b; };
+    v = op(v, dpp_move<kDppQuadXor1>(v));
+    v = op(v, dpp_move<kDppQuadXor2>(v));
+    v = op(v, dpp_move<kDppRowHalfMirror>(v));
+    v = op(v, dpp_move<kDppRowMirror>(v));
     return v;
 }
+
+// Wave sums of N <= 4 values at the cost of about one: the value set is split
+// between the wave halves (exchange with lane ^ 32) and rows (lane ^ 16) as it
+// is reduced, then each row is summed with DPP moves.  Afterwards the lanes of
+// row group k (64 / NP lanes, NP = N rounded up to a power of two) hold the sum
+// of value k.  Fixed order: deterministic.
+template <int N, typename T>
+__device__ __forceinline__ T wave_sum_split(const T (&v)[N], int lane) {
+    static_assert(N >= 1 && N <= 4, "wave_sum_split takes 1 to 4 values");
+    constexpr int NP = N == 1 ? 1 : (N == 2 ? 2 : 4);
+    T a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a[k] = k < N ? v[k] : T(0);
+    const bool up32 = (lane & 32) != 0, up16 = (lane & 16) != 0;
+    T x;
+    if (NP == 4) {
+        // lower half keeps values 0, 1; upper half keeps 2, 3
+        const T s0 = up32 ? a[0] : a[2], s1 = up32 ? a[1] : a[3];
+        const T o0 = up32 ? a[2] : a[0], o1 = up32 ? a[3] : a[1];
+        const T x0 = o0 + __shfl_xor(s0, 32, 64);
+        const T x1 = o1 + __shfl_xor(s1, 32, 64);
+        // the first row of each half keeps x0, the second x1
+        const T s = up16 ? x0 : x1, o = up16 ? x1 : x0;
+        x = o + __shfl_xor(s, 16, 64);
+    } else if (NP == 2) {
+        const T s = up32 ? a[0] : a[1], o = up32 ? a[1] : a[0];
+        x = o + __shfl_xor(s, 32, 64);
+        x = x + __shfl_xor(x, 16, 64);
+    } else {
+        x = a[0] + __shfl_xor(a[0], 32, 64);
+        x = x + __shfl_xor(x, 16, 64);
+    }
+    return row_reduce<false>(x);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_min(T v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) { T w = __shfl_xor(v, o, 64); v = w < v ? w : v; }
-    return v;
+    T w = __shfl_xor(v, 32, 64);
+    v = w < v ? w : v;
+    w = __shfl_xor(v, 16, 64);
+    v = w < v ? w : v;
+    return row_reduce<true>(v);
 }
 
 // Compiler-only memory barrier: loads written above it are issued above it.
@@ -133,13 +214,11 @@ __device__ __forceinline__ T wave_min(T v) {
 // Result valid in thread 0.  lds must hold N*16 entries.
 template <int N, typename T>
 __device__ __forceinline__ void block_sum(T (&v)[N], T* lds) {
+    static_assert(N >= 1 && N <= 4, "block_sum takes 1 to 4 values");
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
-    if (lane == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) lds[k * 16 + w] = v[k];
-    }
+    constexpr int STRIDE = N == 1 ? 64 : (N == 2 ? 32 : 16);
+    const T x = wave_sum_split<N>(v, lane);
+    if ((lane & (STRIDE - 1)) == 0 && lane / STRIDE < N) lds[(lane / STRIDE) * 16 + w] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
 #pragma unroll

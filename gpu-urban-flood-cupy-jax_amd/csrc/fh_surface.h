@@ -101,76 +101,90 @@ __device__ __forceinline__ EdgeFlux<real> edge_flux(real h_l, real hu_l, real hv
     return out;
 }
 
-// Per-member tables of the control kernel.
-struct BeginTables {
-    Seg seg;                 // k_begin blocks of each member
-    const int* cell0;        // [M] first cell of the member
-    const int* cell1;        // [M] end of the member's real cells
+// Per-member forcing tables of the step plan.
+struct ForcingTables {
     const int* rain_off;     // [M] offset of the member's rain intervals
     const int* rain_n;       // [M] number of rain intervals
     const int* ext_off;      // [M] offset of the member's external time table
     const int* ext_n;        // [M] number of external times (0: no external inflow)
-    int single;              // one member: every block belongs to member 0
+    const float* rain_start;
+    const float* rain_end;
+    const float* rain_rate;
+    const float* ext_times;
 };
 
-// ---------------------------------------------------------------------------
-// k_begin: per-member CFL min-reduction (every `interval` steps) + dt policy +
-// forcing lookup.  Wave shuffle -> LDS -> one grid-level pass by the member's
-// last-arriving block.
-// ---------------------------------------------------------------------------
+// The CFL time of one cell (v1_parity.py:153-158): cfl * Lc / (|u| + sqrt(g h)),
+// +inf where that is not finite (so the minimum skips it).
 template <typename real>
-__global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> p,
-        const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc, BeginTables bt,
-        double* __restrict__ part, const float* __restrict__ rain_start, const float* __restrict__ rain_end,
-        const float* __restrict__ rain_rate, const float* __restrict__ ext_times) {
-    __shared__ double lds[16 * 2];
-    const int b = blockIdx.x;
-    const int m = bt.single ? 0 : bt.seg.blk_member[b];
-    Ctl* ctl = ctls + m;
-    const int blk0 = bt.seg.mem_blk0[m], nblk = bt.seg.mem_nblk[m];
-    const bool eval = ctl->eval_next != 0;
-    double vmin_ = INFINITY;
-    if (eval) {
-        real mn = real(INFINITY);
-        const int c1 = bt.cell1[m];
-        for (int c = bt.cell0[m] + (b - blk0) * blockDim.x + threadIdx.x; c < c1; c += nblk * blockDim.x) {
-            const typename V4<real>::type s = state[c];
-            const real hs = vmax(s.x, p.eps);
-            const real u = s.y / hs, v = s.z / hs;
-            real speed = sqrt(u * u + v * v);
-            speed = speed + sqrt(p.g * hs);
-            const real local = p.cfl * lc[c] / vmax(speed, p.tiny);
-            if (isfinite(local) && local < mn) mn = local;
+__device__ __forceinline__ real cell_cfl(const KP<real>& p, const typename V4<real>::type& s, real lc) {
+    const real hs = vmax(s.x, p.eps);
+    const real u = s.y / hs, v = s.z / hs;
+    real speed = sqrt(u * u + v * v);
+    speed = speed + sqrt(p.g * hs);
+    const real local = p.cfl * lc / vmax(speed, p.tiny);
+    return isfinite(local) ? local : real(INFINITY);
+}
+
+// Block minimum, then one atomic min per block on the bits of a non-negative
+// double (which order like the values).
+__device__ __forceinline__ void commit_cfl_min(unsigned long long* bits, double v, double* lds) {
+    v = block_min(v, lds);
+    if (threadIdx.x == 0)
+        __hip_atomic_fetch_min(bits, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Rain rate at time t (parity_runner.py:96-102): rate[lo] when lo =
+// searchsorted(end, t, side="right") < n and t >= start[lo], else 0.  The cache
+// holds the interval of an earlier t of the same clock; with sorted end times the
+// position only grows with t, so the lookup scans forward from there, and is
+// skipped while t stays below the cached end.
+__device__ __forceinline__ double rain_rate_at(const ForcingTables& ft, int m, double t, bool sorted,
+                                               RainCache& rc) {
+    if (!(t < rc.next_end)) {
+        const int n = ft.rain_n[m];
+        const int off = ft.rain_off[m];
+        const float* re = ft.rain_end + off;
+        int lo = 0;
+        if (sorted) {
+            lo = rc.hint < n ? rc.hint : n;
+            while (lo < n && (double)re[lo] <= t) ++lo;
+        } else {
+            int hi = n;
+            while (lo < hi) { const int mid = (lo + hi) >> 1; if ((double)re[mid] <= t) lo = mid + 1; else hi = mid; }
         }
-        vmin_ = (double)mn;
+        rc.hint = lo;
+        rc.start = lo < n ? (double)ft.rain_start[off + lo] : INFINITY;
+        rc.rate = lo < n ? (double)ft.rain_rate[off + lo] : 0.0;
+        rc.next_end = sorted ? (lo < n ? (double)re[lo] : INFINITY) : -INFINITY;
     }
-    if (eval) {
-        vmin_ = block_min(vmin_, lds);
-        if (threadIdx.x == 0) {
-            // non-negative doubles order like their bit patterns: one atomic min per block
-            __hip_atomic_fetch_min(&ctl->cfl_bits, (unsigned long long)__double_as_longlong(vmin_),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
-        if (threadIdx.x != 0) return;
-        __hip_atomic_store(&ctl->counters[C_BEGIN], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    } else if (b != blk0 || threadIdx.x != 0) {
-        return;   // a step that reuses the cached CFL needs only the scalar part below
-    }
-    double gmin = INFINITY;
-    if (eval) {
-        const unsigned long long bits = __hip_atomic_load(&ctl->cfl_bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        gmin = __longlong_as_double((long long)bits);
-        __hip_atomic_store(&ctl->cfl_bits, (unsigned long long)__double_as_longlong(INFINITY), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        ctl->cfl_min = gmin;
-    }
-    const double t = ctl->t;
-    const bool active = (t < ctl->target) && (ctl->interval_step < ctl->max_steps);
-    ctl->active = active ? 1 : 0;
-    if (!active) return;
-    ctl->dt_wet = ctl->pending_wet ? ctl->dt_real : 0.0;
-    ctl->pending_wet = 1;
+    return t >= rc.start ? rc.rate : 0.0;
+}
+
+// The plan of one step of one member: the dt policy of parity_runner.py:124-135
+// and the rain rate of :96-102, from the start-of-step clock.  Every block of
+// the update kernel computes it from the same inputs, so all agree bit for bit.
+template <typename real>
+struct StepPlan {
+    bool active;
+    bool eval_used;          // the CFL minimum of this step was evaluated and used
+    bool cache_evaluated;    // the policy path that updates cached_cfl / cfl_evals
+    double dt;               // (double)(real) dt
+    double rate;             // rain rate
+    double dt_wet;           // pending wet-time increment of the previous step
+    double evaluated;        // the CFL dt this step evaluated or reused
+    RainCache rain;          // rain cache after the lookup at t
+};
+
+template <typename real>
+__device__ __forceinline__ StepPlan<real> plan_step(const Ctl* ctl, const Clock& c, const KP<real>& p,
+                                                    const ForcingTables& ft, int m, double gmin) {
+    StepPlan<real> pl{};
+    const double t = c.t;
+    pl.active = (t < ctl->target) && (c.interval_step < ctl->max_steps);
+    pl.rain = c.rain;
+    if (!pl.active) return pl;
+    pl.dt_wet = c.pending_wet ? c.dt_prev : 0.0;
     double dt;
     double rate;
     if (ctl->override_dt) {
@@ -180,97 +194,155 @@ __global__ void __launch_bounds__(256) k_begin(Ctl* __restrict__ ctls, KP<real> 
         if (p.fixed_dt > 0.0) {
             dt = p.fixed_dt;
         } else {
-            const double evaluated = eval ? gmin : ctl->cached_cfl;
-            const double candidate = eval ? evaluated : (double)((real)evaluated * p.reuse);
+            pl.eval_used = c.eval != 0;
+            pl.evaluated = pl.eval_used ? gmin : c.cached_cfl;
+            const double candidate = pl.eval_used ? pl.evaluated : (double)((real)pl.evaluated * p.reuse);
             dt = fmax(fmin(fmin(candidate, p.max_dt), ctl->final_t - t), p.dt_floor);
-            ctl->cached_cfl = evaluated;
-            if (eval) ctl->cfl_evals += 1;
+            pl.cache_evaluated = true;
         }
-        // parity_runner.py:96-102 -- rain interval containing t
-        const int n_rain = bt.rain_n[m];
-        const float* rs = rain_start + bt.rain_off[m];
-        const float* re = rain_end + bt.rain_off[m];
-        int lo = 0;                    // searchsorted(end, t, side="right")
-        if (ctl->rain_sorted) {
-            // sorted end times: the count of ends <= t only grows with t, so a
-            // forward scan from the position found at an earlier time is exact;
-            // while t stays below the cached next end the position is unchanged
-            if (!(t < ctl->rain_next_end)) {
-                lo = ctl->rain_hint < n_rain ? ctl->rain_hint : n_rain;
-                while (lo < n_rain && (double)re[lo] <= t) ++lo;
-                const int safe = lo < n_rain ? lo : (n_rain > 0 ? n_rain - 1 : 0);
-                ctl->rain_hint = lo;
-                ctl->rain_next_end = lo < n_rain ? (double)re[lo] : INFINITY;
-                ctl->rain_cur_start = n_rain > 0 ? (double)rs[safe] : 0.0;
-                ctl->rain_cur_rate = n_rain > 0 ? (double)rain_rate[bt.rain_off[m] + safe] : 0.0;
-            } else {
-                lo = ctl->rain_hint;
-            }
-            const bool valid = (lo < n_rain) && (t >= ctl->rain_cur_start);
-            rate = valid ? ctl->rain_cur_rate : 0.0;
-        } else {
-            int hi = n_rain;
-            while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)re[mid] <= t) lo = mid + 1; else hi = mid; }
-            const int safe = lo < n_rain ? lo : (n_rain > 0 ? n_rain - 1 : 0);
-            const bool valid = (lo < n_rain) && (t >= (double)rs[safe]);
-            rate = valid ? (double)rain_rate[bt.rain_off[m] + safe] : 0.0;
-        }
+        rate = rain_rate_at(ft, m, t, ctl->rain_sorted != 0, pl.rain);
     }
-    const real dtr = (real)dt;
-    ctl->dt_real = (double)dtr;
-    ctl->dt = (double)dtr;
-    ctl->rain_rate = (double)(real)rate;
-    // parity_runner.py:105-115 -- external inflow interpolation at t + dt/2
-    const int n_ext = bt.ext_n[m];
-    if (n_ext > 0) {
-        const float* et = ext_times + bt.ext_off[m];
-        const double te = t + 0.5 * (double)dtr;
-        int lo = 0;
-        if (ctl->ext_sorted) {         // te = t + dt/2 never decreases along a clock
-            lo = ctl->ext_hint < n_ext ? ctl->ext_hint : n_ext;
-            while (lo < n_ext && (double)et[lo] <= te) ++lo;
-            ctl->ext_hint = lo;
-        } else {
-            int hi = n_ext;
-            while (lo < hi) { int mid = (lo + hi) >> 1; if ((double)et[mid] <= te) lo = mid + 1; else hi = mid; }
-        }
-        const int upper = lo;
-        int lower = upper - 1; lower = lower < 0 ? 0 : (lower > n_ext - 1 ? n_ext - 1 : lower);
-        const int us = upper > n_ext - 1 ? n_ext - 1 : upper;
-        const double den = fmax((double)et[us] - (double)et[lower], 1.0e-12);
-        double frac = (te - (double)et[lower]) / den;
-        frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
-        ctl->ext_lower = lower;
-        ctl->ext_upper = us;
-        ctl->ext_frac = frac;
-    }
-    ctl->t_step = t;
-    ctl->t = t + (double)dtr;
-    ctl->step += 1;
-    ctl->interval_step += 1;
+    pl.dt = (double)(real)dt;
+    pl.rate = (double)(real)rate;
+    return pl;
 }
 
-// Edge index space: Seg plus each member's range of blocks holding ghost edges.
+// Written by one thread of the member's update, after its cells: the step
+// outputs (dt, rain rate, the external inflow interpolation of
+// parity_runner.py:105-115 at t + dt/2, for the drainage kernels) and the clock
+// of the next step (clk[1 - parity]) with its rain interval already looked up,
+// and a fresh CFL minimum slot for that step.
+template <typename real>
+__device__ __forceinline__ void commit_plan(Ctl* ctl, int parity, const Clock& c, const StepPlan<real>& pl,
+                                            const KP<real>& p, const ForcingTables& ft, int m, double gmin) {
+    Clock n = c;
+    ctl->active = pl.active ? 1 : 0;
+    if (pl.active) {
+        ctl->dt = pl.dt;
+        ctl->dt_real = pl.dt;
+        ctl->rain_rate = pl.rate;
+        ctl->t_step = c.t;
+        const int n_ext = ft.ext_n[m];
+        if (n_ext > 0) {
+            const float* et = ft.ext_times + ft.ext_off[m];
+            const double te = c.t + 0.5 * pl.dt;
+            int lo = 0;
+            if (ctl->ext_sorted) {     // te = t + dt/2 never decreases along a clock
+                lo = c.ext_hint < n_ext ? c.ext_hint : n_ext;
+                while (lo < n_ext && (double)et[lo] <= te) ++lo;
+                n.ext_hint = lo;
+            } else {
+                int hi = n_ext;
+                while (lo < hi) { const int mid = (lo + hi) >> 1; if ((double)et[mid] <= te) lo = mid + 1; else hi = mid; }
+            }
+            const int upper = lo;
+            int lower = upper - 1; lower = lower < 0 ? 0 : (lower > n_ext - 1 ? n_ext - 1 : lower);
+            const int us = upper > n_ext - 1 ? n_ext - 1 : upper;
+            const double den = fmax((double)et[us] - (double)et[lower], 1.0e-12);
+            double frac = (te - (double)et[lower]) / den;
+            frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+            ctl->ext_lower = lower;
+            ctl->ext_upper = us;
+            ctl->ext_frac = frac;
+        }
+        if (pl.eval_used) ctl->cfl_min = gmin;
+        n.t = c.t + pl.dt;
+        n.step = c.step + 1;
+        n.interval_step = c.interval_step + 1;
+        if (pl.cache_evaluated) {
+            n.cached_cfl = pl.evaluated;
+            if (pl.eval_used) n.cfl_evals = c.cfl_evals + 1;
+        }
+        n.dt_prev = pl.dt;
+        n.pending_wet = 1;
+        n.rain = pl.rain;
+        if (ctl->rain_sorted) (void)rain_rate_at(ft, m, n.t, true, n.rain);
+        n.eval = (p.fixed_dt <= 0.0 && ctl->override_dt == 0 && (n.step % p.interval) == 0) ? 1 : 0;
+        n.active = (n.t < ctl->target) && (n.interval_step < ctl->max_steps);
+    }
+    ctl->clk[1 - parity] = n;
+    ctl->cfl_bits[1 - parity] = (unsigned long long)__double_as_longlong(INFINITY);
+}
+
+// ---------------------------------------------------------------------------
+// k_cfl: CFL minimum over every cell of each member (stable_timestep,
+// v1_parity.py:153-158); wave shuffle -> LDS -> one atomic min per block, the
+// last-arriving block of the member stores it in cfl_min.
+// ---------------------------------------------------------------------------
+struct CellRanges {
+    Seg seg;                 // k_cfl blocks of each member
+    const int* cell0;        // [M] first cell of the member
+    const int* cell1;        // [M] end of the member's real cells
+};
+
+template <typename real>
+__global__ void __launch_bounds__(256) k_cfl(Ctl* __restrict__ ctls, KP<real> p, int parity,
+        const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc, CellRanges cr) {
+    __shared__ double lds[16 * 2];
+    const int b = blockIdx.x;
+    const int m = cr.seg.blk_member[b];
+    Ctl* ctl = ctls + m;
+    const int blk0 = cr.seg.mem_blk0[m], nblk = cr.seg.mem_nblk[m];
+    real mn = real(INFINITY);
+    const int c1 = cr.cell1[m];
+    for (int c = cr.cell0[m] + (b - blk0) * blockDim.x + threadIdx.x; c < c1; c += nblk * blockDim.x) {
+        const real local = cell_cfl(p, state[c], lc[c]);
+        if (local < mn) mn = local;
+    }
+    // the next step's slot (parity) holds +inf between steps and is left so
+    commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
+    if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
+    if (threadIdx.x != 0) return;
+    __hip_atomic_store(&ctl->counters[C_BEGIN], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long bits = __hip_atomic_load(&ctl->cfl_bits[parity], __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+    ctl->cfl_min = __longlong_as_double((long long)bits);
+    __hip_atomic_store(&ctl->cfl_bits[parity], (unsigned long long)__double_as_longlong(INFINITY), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Edge index space: Seg, plus the blocks after the edge blocks that cover each
+// member's cells which no edge reaches (inactive cells, cells whose every edge
+// was dropped), so a CFL evaluation in the flux kernel sees every cell.
 struct EdgeSeg {
     Seg seg;
-    const int* gblk0;
-    const int* gnblk;
+    int edge_blocks;         // blocks of the edge space; the orphan blocks follow
+    Seg oseg;                // orphan-cell blocks of each member
+    const int* ocells;       // [orphans] cell of each orphan slot
 };
 
 // ---------------------------------------------------------------------------
-// k_flux: EPT edges per thread (tile = blockDim * EPT, coalesced per k).  Each
-// member's edges are laid out interior first, then ghost edges, so only the
-// member's tail blocks reduce boundary volume.  Writes the edge record
-// rec[e] = (m, fuL, fvL, fuR), recR[e] = fvR (all * length).
+// k_flux: EPT edges per thread (tile = blockDim * EPT, coalesced per k).  Writes
+// the edge record rec[e] = (m, fuL, fvL, fuR), recR[e] = fvR (all * length).  On
+// a step that evaluates the CFL condition it also reduces the CFL time of both
+// cells of every edge (and of the orphan cells) into the step's minimum slot:
+// every cell counts at least once, and a cell counted twice gives the same value.
 // ---------------------------------------------------------------------------
 template <typename real, int EPT>
-__global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es,
+__global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
         const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
-        const typename V4<real>::type* __restrict__ state,
-        typename V4<real>::type* __restrict__ rec, real* __restrict__ recR, double* __restrict__ part) {
+        const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc,
+        typename V4<real>::type* __restrict__ rec, real* __restrict__ recR) {
     typedef typename V4<real>::type V;
-    const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    __shared__ double lds[16];
+    if ((int)blockIdx.x >= es.edge_blocks) {
+        // orphan cells: CFL only
+        const int ob = blockIdx.x - es.edge_blocks;
+        const int m = es.oseg.blk_member[ob];
+        Ctl* ctl = ctls + m;
+        const Clock& ck = ctl->clk[parity];
+        if (!ck.active || !ck.eval || ctl->override_dt || p.fixed_dt > 0.0) return;
+        const int o = ob * (int)blockDim.x + threadIdx.x;
+        real mn = real(INFINITY);
+        if (o < es.oseg.mem_end[m]) {
+            const int c = es.ocells[o];
+            mn = cell_cfl(p, state[c], lc[c]);
+        }
+        commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
+        return;
+    }
+    const int b = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
     const int m = es.seg.blk_member[b];
     // every load ahead of the control block: the grid covers the padded edge
     // space exactly, padding edges point at cell 0 as copy ghosts
@@ -289,12 +361,13 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         sls[k] = state[lft[k]];
         srs[k] = state[rgt[k] >= 0 ? rgt[k] : lft[k]];   // ghost edges: unused
     }
+    const int end = es.seg.mem_end[m];
     FH_ISSUE_LOADS_OF("s"(el), "s"(er), "s"(enx), "s"(eny), "s"(elen), "s"(state));
     Ctl* ctl = ctls + m;
-    if (!ctl->active) return;
-    const int end = es.seg.mem_end[m];
-    const real dt = (real)ctl->dt_real;
-    double bin = 0.0, bout = 0.0;
+    const Clock& ck = ctl->clk[parity];
+    if (!ck.active) return;
+    const bool eval = ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);
+    real mn = real(INFINITY);
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
         const int e = (b * EPT + k) * (int)blockDim.x + threadIdx.x;
@@ -317,21 +390,17 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         V o;
         o.x = f.m; o.y = f.fuL; o.z = f.fvL; o.w = f.fuR;
         rec[e] = o;
-        if (r >= 0) {
-            recR[e] = f.fvR;
-        } else {
-            const double outward = (double)(f.m * dt);
-            bin += fmax(-outward, 0.0);
-            bout += fmax(outward, 0.0);
+        if (r >= 0) recR[e] = f.fvR;
+        if (eval) {
+            const real cl = cell_cfl(p, sl, lc[lft[k]]);
+            mn = cl < mn ? cl : mn;
+            if (r >= 0) {
+                const real cr = cell_cfl(p, srs[k], lc[r]);
+                mn = cr < mn ? cr : mn;
+            }
         }
     }
-    const int g0 = es.gblk0[m], gn = es.gnblk[m];
-    if (gn > 0 && b >= g0 && b < g0 + gn) {
-        __shared__ double lds[16 * 2];
-        double v[2] = {bin, bout};
-        block_sum<2>(v, lds);
-        accumulate_slot<2>(part, b, v);
-    }
+    if (eval) commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
 }
 
 // Error-free transformation a + b = s + err (TwoSum, Knuth).
@@ -365,7 +434,8 @@ struct CellSources {
 };
 
 template <typename real, bool COMP, bool CLS>
-__global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real> p, Seg seg,
+__global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
+        ForcingTables ft,
         const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
         const typename V4<real>::type* __restrict__ rec, const real* __restrict__ recR,
         typename V4<real>::type* __restrict__ state, typename V4<real>::type* __restrict__ comp,
@@ -389,7 +459,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int en = ent[k];
-        const int pos = en >= 0 ? (en >> 1) : 0;
+        const int pos = en >= 0 ? (en >> 2) : 0;
         rr[k] = rec[pos];
         rR[k] = recR[pos];
     }
@@ -405,21 +475,30 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
     }
     V cc;
     if (COMP) cc = comp[c];
-    SlotAcc<2> acc2;
-    acc2.prefetch(part, b);
+    SlotAcc<4> acc4;
+    acc4.prefetch(part, b);
+    // the member's block range, read before the barrier (no store can precede
+    // it, so these are scalar loads)
+    const int blk0 = seg.mem_blk0[m], mend = seg.mem_end[m];
     FH_ISSUE_LOADS_OF("s"(state), "s"(wet), "s"(ce0), "s"(ce1), "s"(ce2), "s"(rec), "s"(recR), "s"(area_),
                       "s"(comp), "s"(part));
+    // the step plan (dt policy, rain, inflow interpolation) from the start-of-step
+    // clock, the same in every block; the member's first block commits it
     Ctl* ctl = ctls + m;
-    if (!ctl->active) return;
-    const int blk0 = seg.mem_blk0[m];
-    const real dt = (real)ctl->dt_real;
-    const real dt_wet = (real)ctl->dt_wet;
-    const real rate = (real)ctl->rain_rate;
-    double rain_v = 0.0, inf_v = 0.0;
-    if (b == blk0 && threadIdx.x == 0) {
-        ctl->eval_next = (p.fixed_dt <= 0.0 && ctl->override_dt == 0 && (ctl->step % p.interval) == 0) ? 1 : 0;
+    const Clock ck = ctl->clk[parity];
+    const bool eval = ck.active && ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);
+    const double gmin = eval ? __longlong_as_double((long long)ctl->cfl_bits[parity]) : INFINITY;
+    const StepPlan<real> pl = plan_step<real>(ctl, ck, p, ft, m, gmin);
+    const bool committer = b == blk0 && threadIdx.x == 0;
+    if (!pl.active) {
+        if (committer) commit_plan<real>(ctl, parity, ck, pl, p, ft, m, gmin);
+        return;
     }
-    if (c < seg.mem_end[m]) {
+    const real dt = (real)pl.dt;
+    const real dt_wet = (real)pl.dt_wet;
+    const real rate = (real)pl.rate;
+    double rain_v = 0.0, inf_v = 0.0, bin = 0.0, bout = 0.0;
+    if (c < mend) {
         // deferred wet-time increment of the previous step (h here is its final depth)
         real wt = wt0;
         wt = wt + (s.x > eps ? dt_wet : zero);
@@ -441,6 +520,11 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
                     Rh = Rh + r.x; Ru = Ru + r.w; Rv = Rv + rR[k]; anyR = true;
                 } else {
                     Lh = Lh + (-r.x); Lu = Lu + (-r.y); Lv = Lv + (-r.z);
+                    if (en & 2) {      // boundary edge: outward volume of this step (v1_parity.py:211-213)
+                        const double outward = (double)(r.x * dt);
+                        bin += fmax(-outward, 0.0);
+                        bout += fmax(outward, 0.0);
+                    }
                 }
             }
             const real dh = anyR ? Lh + Rh : Lh;
@@ -498,40 +582,45 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
             inf_v = (double)infil * (double)A;
         }
     }
-    __shared__ double lds[16 * 2];
-    double v[2] = {rain_v, inf_v};
-    block_sum<2>(v, lds);
-    acc2.commit(part, b, v);
+    __shared__ double lds[16 * 4];
+    double v[4] = {rain_v, inf_v, bin, bout};
+    block_sum<4>(v, lds);
+    acc4.commit(part, b, v);
+    if (committer) commit_plan<real>(ctl, parity, ck, pl, p, ft, m, gmin);
 }
 
-// Applies the pending wet-time increment of each member's last executed step.
+// Applies the pending wet-time increment of each member's last executed step
+// (the clock of parity `parity` is the next step's start).
 template <typename real>
-__global__ void k_finalize_wet(const Ctl* __restrict__ ctls, KP<real> p, Seg seg,
+__global__ void k_finalize_wet(const Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
                                const typename V4<real>::type* __restrict__ state, real* __restrict__ wet) {
     const int b = blockIdx.x;
     const int m = seg.blk_member[b];
-    const Ctl* ctl = ctls + m;
-    if (!ctl->pending_wet) return;
+    const Clock& ck = ctls[m].clk[parity];
+    if (!ck.pending_wet) return;
     const int c = b * blockDim.x + threadIdx.x;
     if (c >= seg.mem_end[m]) return;
-    const real dt = (real)ctl->dt_real;
+    const real dt = (real)ck.dt_prev;
     if (state[c].x > p.eps) wet[c] = wet[c] + dt;
 }
 
-__global__ void k_clear_pending(Ctl* ctls, int members) {
+__global__ void k_clear_pending(Ctl* ctls, int members, int parity) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < members) ctls[m].pending_wet = 0;
+    if (m < members) ctls[m].clk[parity].pending_wet = 0;
 }
 
-__global__ void k_set_target(Ctl* ctls, int members, double target, double final_t, int64_t max_steps) {
+// Starts an advance: target, event end and step budget; the clock of the next
+// step (parity `parity`) starts a new budget and decides whether it executes.
+__global__ void k_set_target(Ctl* ctls, int members, int parity, double target, double final_t, int64_t max_steps) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= members) return;
     Ctl* ctl = ctls + m;
     ctl->target = target;
     ctl->final_t = final_t;
     ctl->max_steps = max_steps;
-    ctl->interval_step = 0;
-    ctl->active = 0;
+    Clock& ck = ctl->clk[parity];
+    ck.interval_step = 0;
+    ck.active = (ck.t < target) && (0 < max_steps);
 }
 
 __global__ void k_set_override(Ctl* ctls, int members, int on, double dt, double rain) {
@@ -541,7 +630,6 @@ __global__ void k_set_override(Ctl* ctls, int members, int on, double dt, double
     ctl->override_dt = on;
     ctl->override_dt_value = dt;
     ctl->override_rain = rain;
-    if (on) ctl->eval_next = 0;
 }
 
 // Counts non-finite depths and node volumes over the whole batch
@@ -571,10 +659,10 @@ __global__ void k_finite(Ctl* __restrict__ ctls, int n_cells, const typename V4<
 // in that kernel's index space, N terms per slot, and the MassLedger terms.
 struct FoldGroup {
     double* acc;
-    const int* blk0;    // [M] first block of the member (or first ghost block)
+    const int* blk0;    // [M] first block of the member
     const int* nblk;    // [M] number of blocks
-    int n;              // terms per slot (1..3)
-    int terms[3];
+    int n;              // terms per slot (1..4)
+    int terms[4];
 };
 
 struct FoldTables {
@@ -586,7 +674,7 @@ struct FoldTables {
 // order), writes the sums as the member's diagnostics, adds them to its FP64
 // ledger and clears the slots.
 __global__ void __launch_bounds__(256) k_fold(Ctl* __restrict__ ctls, FoldTables f) {
-    __shared__ double lds[16 * 3];
+    __shared__ double lds[16 * 4];
     const int m = blockIdx.x;
     Ctl* ctl = ctls + m;
     double d[kLedgerTerms];
@@ -594,12 +682,12 @@ __global__ void __launch_bounds__(256) k_fold(Ctl* __restrict__ ctls, FoldTables
     for (int gi = 0; gi < f.ngroups; ++gi) {
         const FoldGroup& g = f.g[gi];
         const int b0 = g.blk0[m], nb = g.nblk[m];
-        double v[3] = {0.0, 0.0, 0.0};
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
         for (int i = threadIdx.x; i < nb; i += blockDim.x) {
             double* slot = g.acc + (size_t)(b0 + i) * g.n;
             for (int k = 0; k < g.n; ++k) { v[k] += slot[k]; slot[k] = 0.0; }
         }
-        block_sum<3>(v, lds);
+        block_sum<4>(v, lds);
         if (threadIdx.x == 0)
             for (int k = 0; k < g.n; ++k) d[g.terms[k]] = v[k];
     }

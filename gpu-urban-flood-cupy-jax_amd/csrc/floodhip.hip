@@ -353,11 +353,14 @@ public:
         FH_CHECK(hipHostMalloc((void**)&ctl_h, sizeof(fh::Ctl) * M, hipHostMallocDefault));
         for (auto& m : mem) {
             std::memset(&m.ctl, 0, sizeof(fh::Ctl));
-            m.ctl.cached_cfl = 1.0;
             const double inf = INFINITY;
-            std::memcpy(&m.ctl.cfl_bits, &inf, sizeof(inf));
-            m.ctl.eval_next = P.fixed_dt > 0.0 ? 0 : 1;
-            m.ctl.rain_next_end = -INFINITY;
+            for (int q = 0; q < 2; ++q) {
+                fh::Clock& ck = m.ctl.clk[q];
+                ck.cached_cfl = 1.0;
+                ck.eval = P.fixed_dt > 0.0 ? 0 : 1;
+                ck.rain.next_end = -INFINITY;
+                std::memcpy(&m.ctl.cfl_bits[q], &inf, sizeof(inf));
+            }
         }
         for (int m = 0; m < M; ++m) ctl_h[m] = mem[m].ctl;
         FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl) * M, hipMemcpyHostToDevice, stream));
@@ -592,9 +595,9 @@ public:
         // forcing-table hints start over; the forward scan is used only on sorted tables
         pull_ctl();
         fh::Ctl& c = ctl_h[cur];
-        c.rain_hint = 0;
-        c.ext_hint = 0;
-        c.rain_next_end = -INFINITY;
+        c.clk[par].rain.hint = 0;
+        c.clk[par].ext_hint = 0;
+        c.clk[par].rain.next_end = -INFINITY;
         c.rain_sorted = std::is_sorted(s.rain_e.begin(), s.rain_e.end()) ? 1 : 0;
         c.ext_sorted = std::is_sorted(s.ext_t.begin(), s.ext_t.end()) ? 1 : 0;
         push_ctl();
@@ -702,22 +705,22 @@ public:
     // ctl_h is indexed by member id (not pack position)
     void set_clock(double t, double cached, int64_t step) override {
         pull_ctl();
-        fh::Ctl& c = ctl_h[cur];
+        fh::Clock& c = ctl_h[cur].clk[par];
         c.t = t;
         c.cached_cfl = (double)(real)cached;
         c.step = step;
-        c.rain_hint = 0;
+        c.rain.hint = 0;
         c.ext_hint = 0;
-        c.rain_next_end = -INFINITY;
-        c.eval_next = (P.fixed_dt <= 0.0 && (step % kp.interval) == 0) ? 1 : 0;
+        c.rain.next_end = -INFINITY;
+        c.eval = (P.fixed_dt <= 0.0 && (step % kp.interval) == 0) ? 1 : 0;
         push_ctl();
     }
     void get_clock(fh_clock* c) override {
         pull_ctl();
-        const fh::Ctl& x = ctl_h[cur];
+        const fh::Clock& x = ctl_h[cur].clk[par];
         c->time_s = x.t;
         c->cached_dt_s = x.cached_cfl;
-        c->last_dt_s = x.dt;
+        c->last_dt_s = ctl_h[cur].dt;
         c->global_step = x.step;
         c->interval_steps = x.interval_step;
         c->cfl_evaluations = x.cfl_evals;
@@ -840,7 +843,6 @@ public:
         // ---------------- surface arrays
         std::vector<int> el(edges.total, 0), er(edges.total, fh::kGhostCopy);
         std::vector<real> nx(edges.total, 0), ny(edges.total, 0), len(edges.total, 0);
-        std::vector<int> gblk0(M, 0), gnblk(M, 0);
         std::vector<int> ce0(cells.total, fh::kCellInactive), ce1(cells.total, -1), ce2(cells.total, -1);
         std::vector<V> st(cells.total);
         std::vector<real> wet(cells.total, 0), area(cells.total, 1), n2(cells.total, 0), lc(cells.total, 0),
@@ -858,15 +860,13 @@ public:
                 er[q] = m.er[e] >= 0 ? m.cp(m.er[e]) + cb : m.er[e];
                 nx[q] = m.nx[e]; ny[q] = m.ny[e]; len[q] = m.len[e];
             }
-            const int tile = edges.tile;
-            if (m.E > m.E_int) {
-                gblk0[i] = (eb + (int)m.E_int) / tile;
-                gnblk[i] = (eb + (int)m.E - 1) / tile + 1 - gblk0[i];
-            }
+            // entry = (edge position << 2) | (boundary edge << 1) | right side
             auto enc = [&](int x) {
                 if (x < 0) return x;
-                const int p = perm ? m.epos[(size_t)(x >> 1)] : (x >> 1);
-                return ((p + eb) << 1) | (x & 1);
+                const int staged = x >> 1;
+                const int p = perm ? m.epos[(size_t)staged] : staged;
+                const int ghost = staged >= m.E_int ? 1 : 0;
+                return ((p + eb) << 2) | (ghost << 1) | (x & 1);
             };
             for (int64_t c = 0; c < m.C; ++c) {
                 const int64_t q = cb + m.cp((int)c);
@@ -896,13 +896,29 @@ public:
         }
         d_rain_s.set(rs_all, s); d_rain_e.set(re_all, s); d_rain_r.set(rr_all, s); d_ext_t.set(et_all, s);
         seg_cells.set(cells, order, s); seg_edges.set(edges, order, s); seg_begin.set(begin_l, order, s);
-        d_gblk0.set(by_id(gblk0, order), s); d_gnblk.set(by_id(gnblk, order), s);
         d_cell0.set(by_id(cell0, order), s); d_cell1.set(by_id(cell1, order), s);
         d_rain_off.set(by_id(rain_off, order), s); d_rain_n.set(by_id(rain_n, order), s);
         d_ext_off.set(by_id(ext_off, order), s); d_ext_n.set(by_id(ext_n, order), s);
-        d_part_begin.alloc(std::max(begin_l.blocks, 1));
-        d_part_flux.set(std::vector<double>(2 * (size_t)std::max(edges.blocks, 1), 0.0), s);
-        d_part_upd.set(std::vector<double>(2 * (size_t)std::max(cells.blocks, 1), 0.0), s);
+        d_part_upd.set(std::vector<double>(4 * (size_t)std::max(cells.blocks, 1), 0.0), s);
+        // cells no kept edge reaches: the flux kernel's CFL evaluation covers them in extra blocks
+        {
+            std::vector<std::vector<int>> orph(M);
+            for (int i = 0; i < M; ++i) {
+                const Member<real>& m = mem[order[i]];
+                std::vector<char> seen((size_t)m.C, 0);
+                for (int64_t e = 0; e < m.E; ++e) {
+                    seen[(size_t)m.cp(m.el[e])] = 1;
+                    if (m.er[e] >= 0) seen[(size_t)m.cp(m.er[e])] = 1;
+                }
+                for (int64_t c = 0; c < m.C; ++c) if (!seen[(size_t)c]) orph[i].push_back((int)c + cells.base[i]);
+            }
+            orphans.build(counts_of(orph), 256);
+            std::vector<int> oc((size_t)orphans.total, 0);
+            for (int i = 0; i < M; ++i)
+                for (size_t k = 0; k < orph[i].size(); ++k) oc[(size_t)orphans.base[i] + k] = orph[i][k];
+            d_ocells.set(oc, s);
+            seg_orph.set(orphans, order, s);
+        }
         d_part_fin.alloc(1024);
         pack_drainage(icells, rcells, xcells, area);
         // control blocks: per member, unchanged
@@ -1278,9 +1294,9 @@ public:
     }
 
     // ------------------------------------------------------------------ stepping
-    fh::BeginTables begin_tables() const {
-        return fh::BeginTables{seg_begin.seg(), d_cell0.p, d_cell1.p, d_rain_off.p, d_rain_n.p, d_ext_off.p, d_ext_n.p,
-                               M == 1 ? 1 : 0};
+    fh::ForcingTables forcing_tables() const {
+        return fh::ForcingTables{d_rain_off.p, d_rain_n.p, d_ext_off.p, d_ext_n.p,
+                                 d_rain_s.p, d_rain_e.p, d_rain_r.p, d_ext_t.p};
     }
 
     // Stage launches go through here.  While a kernel is being timed the launch
@@ -1296,16 +1312,18 @@ public:
             hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, static_cast<KA>(args)...);
     }
 
-    void launch_begin() {
-        go(fh::k_begin<real>, begin_l.blocks, 256, ctl, kp, d_state.p, d_lc.p, begin_tables(), d_part_begin.p,
-           d_rain_s.p, d_rain_e.p, d_rain_r.p, d_ext_t.p);
+    // CFL minimum over every cell of each member into cfl_min (stable_timestep)
+    void launch_cfl() {
+        go(fh::k_cfl<real>, begin_l.blocks, 256, ctl, kp, par, d_state.p, d_lc.p,
+           fh::CellRanges{seg_begin.seg(), d_cell0.p, d_cell1.p});
     }
     void launch_flux() {
-        if (edges.blocks == 0) return;
-        fh::EdgeSeg es{seg_edges.seg(), d_gblk0.p, d_gnblk.p};
-#define FH_FLUX(EPT_)                                                                                  \
-        go(fh::k_flux<real, EPT_>, edges.blocks, bs, ctl, kp, es, d_el.p, d_er.p, d_nx.p, d_ny.p, d_len.p, \
-           d_state.p, d_rec.p, d_recR.p, d_part_flux.p)
+        if (edges.blocks + orphans.blocks == 0) return;
+        fh::EdgeSeg es{seg_edges.seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
+        const unsigned grid = (unsigned)(edges.blocks + orphans.blocks);
+#define FH_FLUX(EPT_)                                                                                    \
+        go(fh::k_flux<real, EPT_>, grid, bs, ctl, kp, es, par, d_el.p, d_er.p, d_nx.p, d_ny.p, d_len.p, \
+           d_state.p, d_lc.p, d_rec.p, d_recR.p)
         if (ept == 4) FH_FLUX(4);
         else if (ept == 2) FH_FLUX(2);
         else FH_FLUX(1);
@@ -1314,7 +1332,8 @@ public:
     void launch_update() {
         const fh::CellSources<real> src{d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p, d_cls.p, d_cls_tab.p, d_cls_hk.p};
 #define FH_UPDATE(COMP_, CLS_)                                                                              \
-        go(fh::k_update<real, COMP_, CLS_>, cells.blocks, bs, ctl, kp, seg_cells.seg(), d_ce0.p, d_ce1.p, d_ce2.p, \
+        go(fh::k_update<real, COMP_, CLS_>, cells.blocks, bs, ctl, kp, seg_cells.seg(), par, forcing_tables(),    \
+           d_ce0.p, d_ce1.p, d_ce2.p,                                                                           \
            d_rec.p, d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, src, d_part_upd.p)
         if (P.compensated) {
             if (use_classes) FH_UPDATE(true, true); else FH_UPDATE(true, false);
@@ -1358,11 +1377,13 @@ public:
             });
     }
 
+    // One step: flux and update on the clock of parity `par`; the update writes
+    // the next step's clock into the other slot, so the parity flips per step.
     void launch_step() {
-        launch_begin();
         launch_flux();
         launch_update();
         launch_drainage([](const char*, double, auto&& fn) { fn(); });
+        par ^= 1;
     }
 
     // Two captured step graphs: the full chunk and a short one for the last
@@ -1372,32 +1393,36 @@ public:
         hipGraphExec_t exec = nullptr;
         int steps = 0;
     };
-    StepGraph graphs[2];
+    StepGraph graphs[2][2];   // [full / short][parity of the first step]
 
     void drop_graph() {
-        for (auto& g : graphs) {
-            if (g.exec) (void)hipGraphExecDestroy(g.exec);
-            if (g.graph) (void)hipGraphDestroy(g.graph);
-            g = StepGraph{};
-        }
+        for (auto& row : graphs)
+            for (auto& g : row) {
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                if (g.graph) (void)hipGraphDestroy(g.graph);
+                g = StepGraph{};
+            }
     }
 
     void build_graph(StepGraph& g, int k) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (g.graph) (void)hipGraphDestroy(g.graph);
         g = StepGraph{};
+        const int p0 = par;
         FH_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
         for (int i = 0; i < k; ++i) launch_step();
         FH_CHECK(hipStreamEndCapture(stream, &g.graph));
+        par = p0;   // capture only recorded the steps
         FH_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
         g.steps = k;
     }
 
     void launch_chunk(int k) {
         if (P.use_graph) {
-            StepGraph& g = graphs[k == std::max(1, P.steps_per_launch) ? 0 : 1];
+            StepGraph& g = graphs[k == std::max(1, P.steps_per_launch) ? 0 : 1][par];
             if (!g.exec || g.steps != k) build_graph(g, k);
             FH_CHECK(hipGraphLaunch(g.exec, stream));
+            par ^= (k & 1);
         } else {
             for (int i = 0; i < k; ++i) launch_step();
         }
@@ -1407,22 +1432,22 @@ public:
     void fold() {
         fh::FoldTables f{};
         auto group = [&](int i, double* acc, const DevSeg& seg, int n, int t0, int t1, int t2) {
-            f.g[i] = fh::FoldGroup{acc, seg.mem_blk0.p, seg.mem_nblk.p, n, {t0, t1, t2}};
+            f.g[i] = fh::FoldGroup{acc, seg.mem_blk0.p, seg.mem_nblk.p, n, {t0, t1, t2, 0}};
         };
-        group(0, d_part_upd.p, seg_cells, 2, fh::L_RAIN, fh::L_INFIL, 0);
-        f.g[1] = fh::FoldGroup{d_part_flux.p, d_gblk0.p, d_gnblk.p, 2, {fh::L_BIN, fh::L_BOUT, 0}};
-        group(2, d_part_inlet.p, seg_icl, 2, fh::L_CAPTURE, fh::L_SURCHARGE, 0);
-        group(3, d_part_node.p, seg_nodes, 3, fh::L_OUTFALL, fh::L_OVERFLOW, fh::L_UNRETURNED);
-        group(4, d_part_ret.p, seg_rcl, 2, fh::L_RCAPTURE, fh::L_RLOSS, 0);
-        group(5, d_part_ext.p, seg_xcl, 1, fh::L_EXTERNAL, 0, 0);
-        f.ngroups = 6;
+        f.g[0] = fh::FoldGroup{d_part_upd.p, seg_cells.mem_blk0.p, seg_cells.mem_nblk.p, 4,
+                               {fh::L_RAIN, fh::L_INFIL, fh::L_BIN, fh::L_BOUT}};
+        group(1, d_part_inlet.p, seg_icl, 2, fh::L_CAPTURE, fh::L_SURCHARGE, 0);
+        group(2, d_part_node.p, seg_nodes, 3, fh::L_OUTFALL, fh::L_OVERFLOW, fh::L_UNRETURNED);
+        group(3, d_part_ret.p, seg_rcl, 2, fh::L_RCAPTURE, fh::L_RLOSS, 0);
+        group(4, d_part_ext.p, seg_xcl, 1, fh::L_EXTERNAL, 0, 0);
+        f.ngroups = 5;
         fh::k_fold<<<M, 256, 0, stream>>>(ctl, f);
     }
 
     void finalize_pending() {
-        fh::k_finalize_wet<real><<<std::max(cells.blocks, 1), bs, 0, stream>>>(ctl, kp, seg_cells.seg(), d_state.p,
-                                                                               d_wet.p);
-        fh::k_clear_pending<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M);
+        fh::k_finalize_wet<real><<<std::max(cells.blocks, 1), bs, 0, stream>>>(ctl, kp, seg_cells.seg(), par,
+                                                                               d_state.p, d_wet.p);
+        fh::k_clear_pending<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, par);
     }
 
     bool check_finite() {
@@ -1434,14 +1459,14 @@ public:
 
     bool any_running(double target, int64_t max_steps) const {
         for (int m = 0; m < M; ++m)
-            if (ctl_h[m].t < target && ctl_h[m].interval_step < max_steps) return true;
+            if (ctl_h[m].clk[par].t < target && ctl_h[m].clk[par].interval_step < max_steps) return true;
         return false;
     }
 
     int advance_to(double target, double final_t, int64_t max_steps) override {
         ensure_packed();
         if (max_steps <= 0) throw ValueError{"max_steps must be positive"};
-        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, target, final_t, max_steps);
+        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, par, target, final_t, max_steps);
         const int K = std::max(1, P.steps_per_launch);
         pull_ctl();
         while (any_running(target, max_steps)) {
@@ -1449,10 +1474,11 @@ public:
             double min_rem = 1e300;
             int64_t budget_left = 0;
             for (int m = 0; m < M; ++m) {
-                if (!(ctl_h[m].t < target && ctl_h[m].interval_step < max_steps)) continue;
+                const fh::Clock& ck = ctl_h[m].clk[par];
+                if (!(ck.t < target && ck.interval_step < max_steps)) continue;
                 const double dt_est = std::max(ctl_h[m].dt > 0 ? ctl_h[m].dt : 1e-3, 1e-6);
-                min_rem = std::min(min_rem, (target - ctl_h[m].t) / dt_est);
-                budget_left = std::max<int64_t>(budget_left, max_steps - ctl_h[m].interval_step);
+                min_rem = std::min(min_rem, (target - ck.t) / dt_est);
+                budget_left = std::max<int64_t>(budget_left, max_steps - ck.interval_step);
             }
             const int Kt = std::min(K, kTailSteps);
             const double rem = std::min(min_rem, (double)budget_left);
@@ -1474,7 +1500,7 @@ public:
         const bool finite = check_finite();
         if (!finite) return FH_ERR_SIMULATION;
         for (int m = 0; m < M; ++m)
-            if (ctl_h[m].t < target) return FH_ERR_BUDGET;
+            if (ctl_h[m].clk[par].t < target) return FH_ERR_BUDGET;
         return FH_OK;
     }
 
@@ -1492,7 +1518,7 @@ public:
             dd.use_override = 1;
         }
         fold();   // commits anything pending, so the next fold holds this step alone
-        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, INFINITY, INFINITY, 1);
+        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, par, INFINITY, INFINITY, 1);
         fh::k_set_override<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, 1, dt, rain);
         launch_step();
         fh::k_set_override<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, 0, 0.0, 0.0);
@@ -1500,8 +1526,10 @@ public:
         finalize_pending();
         dd.use_override = 0;
         pull_ctl();
-        for (int m = 0; m < M; ++m)
-            ctl_h[m].eval_next = (P.fixed_dt <= 0.0 && (ctl_h[m].step % kp.interval) == 0) ? 1 : 0;
+        for (int m = 0; m < M; ++m) {
+            fh::Clock& ck = ctl_h[m].clk[par];
+            ck.eval = (P.fixed_dt <= 0.0 && (ck.step % kp.interval) == 0) ? 1 : 0;
+        }
         push_ctl();
         if (diag)
             for (int k = 0; k < fh::kLedgerTerms; ++k) diag[k] = ctl_h[cur].diag[k];
@@ -1509,25 +1537,9 @@ public:
 
     double stable_timestep() override {
         ensure_packed();
-        // run k_begin's reduction on inactive steps (target = -inf): the clock is untouched
+        launch_cfl();
         pull_ctl();
-        std::vector<fh::Ctl> saved(ctl_h, ctl_h + M);
-        for (int m = 0; m < M; ++m) {
-            ctl_h[m].eval_next = 1;
-            ctl_h[m].override_dt = 0;
-            ctl_h[m].target = -INFINITY;
-        }
-        push_ctl();
-        launch_begin();
-        pull_ctl();
-        const double r = ctl_h[cur].cfl_min;
-        for (int m = 0; m < M; ++m) {
-            const double cm_ = ctl_h[m].cfl_min;
-            ctl_h[m] = saved[m];
-            ctl_h[m].cfl_min = cm_;
-        }
-        push_ctl();
-        return r;
+        return ctl_h[cur].cfl_min;
     }
 
     std::vector<KernelTiming> profile(int steps) override {
@@ -1562,12 +1574,12 @@ public:
         const double flux_bytes = Et * (2 * I4 + 3 * R4) + Ct * 4 * R4 + Et * 5 * R4;
         const double upd_bytes = Et * 5 * R4 + Ct * (3 * I4 + 8 * R4 + 2 * R4 + 4 * R4 + 2 * R4) +
                                  (P.compensated ? Ct * 8 * R4 : 0.0);
-        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, INFINITY, INFINITY, (int64_t)steps + 8);
+        fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, par, INFINITY, INFINITY, (int64_t)steps + 8);
         for (int i = 0; i < steps; ++i) {
-            timed("begin", Ct * 4 * R4 / kp.interval, [&] { launch_begin(); });
             timed("flux", flux_bytes, [&] { launch_flux(); });
             timed("update", upd_bytes, [&] { launch_update(); });
             launch_drainage(timed);
+            par ^= 1;
         }
         fold();
         finalize_pending();
@@ -1598,11 +1610,12 @@ public:
     fh::Ctl* ctl = nullptr;     // device, [M] indexed by member id
     fh::Ctl* ctl_h = nullptr;   // pinned host mirror
     int bs = 256, ept = 1;
+    int par = 0;                // parity of the clock slot the next step runs on
     bool packed = false, dirty = true, any_drainage = false;
     int n_receivers = 0;
-    Layout cells, edges, nodes, links, icl, rcl, xcl, begin_l;
-    DevSeg seg_cells, seg_edges, seg_begin, seg_nodes, seg_links, seg_icl, seg_rcl, seg_xcl;
-    DevArray<int> d_gblk0, d_gnblk, d_cell0, d_cell1, d_rain_off, d_rain_n, d_ext_off, d_ext_n;
+    Layout cells, edges, nodes, links, icl, rcl, xcl, begin_l, orphans;
+    DevSeg seg_cells, seg_edges, seg_begin, seg_nodes, seg_links, seg_icl, seg_rcl, seg_xcl, seg_orph;
+    DevArray<int> d_cell0, d_cell1, d_rain_off, d_rain_n, d_ext_off, d_ext_n, d_ocells;
     DevArray<int> d_el, d_er, d_ce0, d_ce1, d_ce2;
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
     DevArray<unsigned short> d_cls;   // source class per cell (when use_classes)
@@ -1610,7 +1623,7 @@ public:
     DevArray<real> d_cls_hk;          // class -> k
     bool use_classes = false;
     DevArray<V> d_rec, d_state, d_comp;
-    DevArray<double> d_part_begin, d_part_flux, d_part_upd, d_part_fin, d_part_inlet, d_part_node, d_part_ret,
+    DevArray<double> d_part_upd, d_part_fin, d_part_inlet, d_part_node, d_part_ret,
         d_part_ext;
     DevArray<float> d_rain_s, d_rain_e, d_rain_r, d_ext_t, d_ext_q, d_ext_ov;
     // drainage
