@@ -44,13 +44,25 @@ struct DrainDev {
     // Row-aligned copies of each row's first table entries (0 where the row has
     // none), loaded together with the row itself rather than through its CSR
     // pointer, which saves one dependent memory round trip per kernel.
-    const int* nf_cell;              // [N] first inlet of the node: cell, surcharge limit
-    const real* nf_surq;
-    const int *icf_node, *icf_slot;  // [inlet cells] first inlet: node, node slot, capture limit,
-    const real *icf_capq, *icf_inv, *icf_As;   //   node invert and storage area
-    const int *lk_a0, *lk_a1, *lk_b0, *lk_b1;  // [L] inlet-slot ranges of the two end nodes,
-    const real *lk_inva, *lk_Asa, *lk_invb, *lk_Asb;   //   their inverts and storage areas
+    // per inlet entry (ic_* order): its cell, its node's invert and storage area,
+    // and the inlets of its node ("siblings", in slot order): first slot, count,
+    // and the first KB siblings' cells and surcharge limits ([KB][entries]; the
+    // entry's own cell and 0 past the count)
+    const int *ie_cell, *ie_row, *ie_s0, *ie_ns, *ie_sc;
+    const real *ie_inv, *ie_As, *ie_sq;
+    real* ic_h;                      // [inlet cells] depth after the inlet exchange of this step
     const int* nle;                  // [KB][N] first KB entries of nl_ent
+    // Link work items of each block of kNodeBlock nodes (k_link_node): the links
+    // whose 'from' node is in the block, then the links entering it from other
+    // blocks, in the block's entry order; lb_ptr [node blocks + 1] delimits them.
+    // Per item: link, end nodes and their inlet-slot ranges.  nl_src gives each
+    // nl_ent entry the block-local item holding its link's request, nls [KB][N]
+    // the same for the first KB entries of each node.
+    const int *lb_ptr, *it_l, *it_a, *it_b, *it_a0, *it_a1, *it_b0, *it_b1;
+    const int *nl_src, *nls;
+    // static link data, two vectors per link: {invert, diameter, length, Manning n},
+    // {minor loss, capacity, rectangle width, rectangle height}
+    const typename V4<real>::type *lk_a, *lk_b;
     const int *nl2e, *nl2o;          // [KB][N] first KB entries of nl2_ent / nl2_oth
     const real* nesw0;               // [N] first entry weight
     const int* rcf_node;             // [return cells] first entry: node, weight; first fallback node
@@ -103,6 +115,13 @@ __device__ __forceinline__ int ahead(int k, int k1) { return k < k1 ? k : 0; }
 
 // entries per row kept in the row-aligned copies of the node->link tables
 constexpr int kRowEntries = 4;
+// inlet cells per block of k_inlet (the inlet-cell layout's tile) and its threads
+constexpr int kInletRows = 256;
+constexpr int kInletThreads = 512;
+// nodes per block of the node kernels (the node layout's tile), and threads
+// per block of k_link_node (one link item each)
+constexpr int kNodeBlock = 256;
+constexpr int kLinkThreads = 512;
 
 // commit helper used by the last block of each ledger kernel
 template <int NT>
@@ -165,104 +184,133 @@ __global__ void __launch_bounds__(256) k_external(Ctl* __restrict__ ctls, KP<rea
 // active: the grid covers exactly the padded index space, so those loads stay
 // in bounds, and the gather chains start one memory round trip earlier.
 
-// Inlet exchange, pass A (one thread per node): surcharge requests of the node's
-// inlets and the node's donor factor  min(1, V / sum(requests)).
+// Surcharge request of one inlet (v1_parity.py:275-283): min(max(-q, 0) dt,
+// max(limit, 0) dt), q the orifice flow from the cell's water surface to the
+// node head.
 template <typename real>
-__global__ void __launch_bounds__(256) k_inlet_node_factor(const Ctl* __restrict__ ctls, KP<real> p,
-        DrainDev<real> d, const typename V4<real>::type* __restrict__ state) {
-    typedef typename V4<real>::type V;
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int mem = d.seg_node.blk_member[blockIdx.x];
-    const int k0 = d.ni_ptr[n], k1 = d.ni_ptr[n + 1];
-    if (k0 == k1) return;
-    const real vol = d.V[n];
-    const real inv = d.inv[n], As = d.As[n];
-    const real sq0 = d.nf_surq[n];                  // first inlet, loaded ahead
-    const V s0 = state[d.nf_cell[n]];
-    FH_ISSUE_LOADS_OF("s"(state));
-    const Ctl* ctl = ctls + mem;
-    if (!ctl->active) return;
-    const real dt = (real)ctl->dt_real;
-    const real nh = head_of(inv, As, vol);
-    auto request = [&](const V& s, real sq) {
-        const real q = orifice_q(p, (s.w + s.x) - nh);
-        return vmin(vmax(-q, real(0)) * dt, vmax(sq, real(0)) * dt);
-    };
-    real total = real(0);
-    {
-        const real rs = request(s0, sq0);
-        d.ni_sur[k0] = rs;
-        total = total + rs;
-    }
-    for (int k = k0 + 1; k < k1; ++k) {
-        const real rs = request(state[d.ni_cell[k]], d.ni_surq[k]);
-        d.ni_sur[k] = rs;
-        total = total + rs;
-    }
-    const real factor = vmin(real(1), vol / vmax(total, real(1.0e-12)));
-    for (int k = k0; k < k1; ++k) d.ni_sur[k] = d.ni_sur[k] * factor;
+__device__ __forceinline__ real surcharge_request(const KP<real>& p, const typename V4<real>::type& s, real sq,
+                                                  real nh, real dt) {
+    const real q = orifice_q(p, (s.w + s.x) - nh);
+    return vmin(vmax(-q, real(0)) * dt, vmax(sq, real(0)) * dt);
 }
 
-// Inlet exchange, pass B (one thread per inlet cell): capture requests, the
-// cell's donor factor min(1, h A / sum), captured volumes, and the depth update
-// h = max(h + sum((sur - cap) / A), 0).  Each inlet's net exchange
-// (capture - surcharge) goes to its node slot for the node volume update.
+// Inlet exchange, v1_parity.py:270-302, one block per kInletRows inlet cells
+// with kInletThreads threads.  Each thread takes one inlet of the block's cells
+// (the ic_* entries of those rows, contiguous): the inlet's surcharge request
+// scaled by its node's donor factor min(1, V / sum of the requests of all the
+// node's inlets, in slot order -- each recomputed from its cell), and its
+// capture request, into LDS.  After the barrier one thread per cell forms the
+// cell's donor factor min(1, h A / sum of its capture requests), the depth
+// h = max(h + sum((sur - cap) / A), 0) (to ic_h: the state keeps the depth the
+// sibling requests read until k_node_update applies it) and the ledger sums;
+// after a second barrier each inlet stores its net exchange (capture -
+// surcharge) in its node slot.  Dynamic LDS: two reals per inlet of the block
+// and one per cell.
 template <typename real>
-__global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
-        typename V4<real>::type* __restrict__ state, const real* __restrict__ area, double* __restrict__ part) {
+__global__ void __launch_bounds__(kInletThreads) k_inlet(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
+        const typename V4<real>::type* __restrict__ state, double* __restrict__ part) {
     typedef typename V4<real>::type V;
+    constexpr int KB = kRowEntries;
+    extern __shared__ __align__(16) unsigned char fh_lds_dyn[];
     const int b = blockIdx.x;
-    const int j = b * blockDim.x + threadIdx.x;
+    const int t = threadIdx.x;
     const int mem = d.seg_icell.blk_member[b];
-    const int c = d.ic_cells[j];
+    const int mend = d.seg_icell.mem_end[mem];
+    const int nent = d.ic_ptr[d.Ci];              // inlet entries of the batch
+    const int e0 = d.ic_ptr[b * kInletRows], e1 = d.ic_ptr[b * kInletRows + kInletRows];
+    real* lsur = reinterpret_cast<real*>(fh_lds_dyn);
+    real* lrc = lsur + (e1 - e0);
+    real* lfac = lrc + (e1 - e0);
+    // this thread's inlet, loaded ahead (threads past the block's inlets read
+    // entry 0, never used)
+    const int e = e0 + t < e1 ? e0 + t : 0;
+    const int n = d.ic_node[e], slot = d.ic_slot[e];
+    const real capq = d.ic_capq[e], inv = d.ie_inv[e], As = d.ie_As[e];
+    const int sib0 = d.ie_s0[e], nsib = d.ie_ns[e];
+    const V sc = state[d.ie_cell[e]];
+    const real Vn = d.V[n];
+    V ss[KB];
+    real sq[KB];
+#pragma unroll
+    for (int u = 0; u < KB; ++u) {
+        ss[u] = state[d.ie_sc[u * nent + e]];
+        sq[u] = d.ie_sq[u * nent + e];
+    }
+    // this thread's cell (threads < kInletRows)
+    const bool row_thread = t < kInletRows;
+    const int j = b * kInletRows + (row_thread ? t : 0);
     const int k0 = d.ic_ptr[j], k1 = d.ic_ptr[j + 1];
-    V s = state[c];
+    const real h0 = state[d.ic_cells[j]].x;
     const real A = d.ic_area[j];
-    // first inlet of the cell, loaded ahead (rows without one read node 0 / slot 0, never used)
-    const int n0 = d.icf_node[j], slot0 = d.icf_slot[j];
-    const real capq0 = d.icf_capq[j], inv0 = d.icf_inv[j], As0 = d.icf_As[j];
-    const real V0 = d.V[n0], sur0 = d.ni_sur[slot0];
     SlotAcc<2> acc2;
     acc2.prefetch(part, b);
-    FH_ISSUE_LOADS_OF("s"(state), "s"(area), "s"(part));
+    FH_ISSUE_LOADS_OF("s"(state), "s"(part));
     Ctl* ctl = ctls + mem;
-    if (!ctl->active) return;
-    double cap_sum = 0.0, sur_sum = 0.0;
-    if (j < d.seg_icell.mem_end[mem]) {
-        const real dt = (real)ctl->dt_real;
-        const real sh = s.w + s.x;
-        real total = real(0);
-        real rc0 = real(0);
-        for (int k = k0; k < k1; ++k) {
-            real nh;
-            real cq;
-            if (k == k0) {
-                nh = head_of(inv0, As0, V0);
-                cq = capq0;
-            } else {
-                const int n = d.ic_node[k];
-                nh = head_of(d.inv[n], d.As[n], d.V[n]);
-                cq = d.ic_capq[k];
-            }
-            const real q = orifice_q(p, sh - nh);
-            const real rc = vmin(vmax(q, real(0)) * dt, vmax(cq, real(0)) * dt);
-            if (k == k0) rc0 = rc;
-            else d.ni_net[d.ic_slot[k]] = rc;          // request, scaled below
-            total = total + rc;
+    if (!ctl->active) return;                 // the whole block: one member
+    const real dt = (real)ctl->dt_real;
+    const bool live_e = e0 + t < e1;
+    if (live_e) {
+        const real nh = head_of(inv, As, Vn);
+        real total = real(0), own = real(0);
+#pragma unroll
+        for (int u = 0; u < KB; ++u) {
+            if (u >= nsib) break;
+            const real rs = surcharge_request(p, ss[u], sq[u], nh, dt);
+            total = total + rs;
+            if (sib0 + u == slot) own = rs;
         }
-        const real factor = vmin(real(1), (s.x * A) / vmax(total, real(1.0e-12)));
+        for (int u = KB; u < nsib; ++u) {
+            const real rs = surcharge_request(p, state[d.ni_cell[sib0 + u]], d.ni_surq[sib0 + u], nh, dt);
+            total = total + rs;
+            if (sib0 + u == slot) own = rs;
+        }
+        lsur[t] = own * vmin(real(1), Vn / vmax(total, real(1.0e-12)));
+        const real q = orifice_q(p, (sc.w + sc.x) - nh);
+        lrc[t] = vmin(vmax(q, real(0)) * dt, vmax(capq, real(0)) * dt);
+    }
+    for (int i = e0 + t + kInletThreads; i < e1; i += kInletThreads) {   // blocks with more inlets than threads
+        const int ni = d.ic_node[i], si = d.ic_slot[i];
+        const real Vi = d.V[ni];
+        const real nh = head_of(d.ie_inv[i], d.ie_As[i], Vi);
+        real total = real(0), own = real(0);
+        for (int u = 0; u < d.ie_ns[i]; ++u) {
+            const int k = d.ie_s0[i] + u;
+            const real rs = surcharge_request(p, state[d.ni_cell[k]], d.ni_surq[k], nh, dt);
+            total = total + rs;
+            if (k == si) own = rs;
+        }
+        lsur[i - e0] = own * vmin(real(1), Vi / vmax(total, real(1.0e-12)));
+        const V si_s = state[d.ie_cell[i]];
+        const real q = orifice_q(p, (si_s.w + si_s.x) - nh);
+        lrc[i - e0] = vmin(vmax(q, real(0)) * dt, vmax(d.ic_capq[i], real(0)) * dt);
+    }
+    __syncthreads();
+    double cap_sum = 0.0, sur_sum = 0.0;
+    if (row_thread && j < mend) {
+        real total = real(0);
+        for (int k = k0; k < k1; ++k) total = total + lrc[k - e0];
+        const real factor = vmin(real(1), (h0 * A) / vmax(total, real(1.0e-12)));
         real acc = real(0);
         for (int k = k0; k < k1; ++k) {
-            const int slot = k == k0 ? slot0 : d.ic_slot[k];
-            const real cp = (k == k0 ? rc0 : d.ni_net[slot]) * factor;
-            const real sr = k == k0 ? sur0 : d.ni_sur[slot];
-            d.ni_net[slot] = cp - sr;
+            const real cp = lrc[k - e0] * factor;
+            const real sr = lsur[k - e0];
             acc = acc + (sr - cp) / A;
             cap_sum += (double)cp;
             sur_sum += (double)sr;
         }
-        s.x = vmax(s.x + acc, real(0));
-        state[c].x = s.x;
+        lfac[t] = factor;
+        d.ic_h[j] = vmax(h0 + acc, real(0));
+    }
+    __syncthreads();
+    // each inlet's net exchange, with its cell's factor (the entries of row r
+    // are ic_ptr[r] .. ic_ptr[r + 1], so the row is found from the entry)
+    if (live_e) {
+        const int r = d.ie_row[e] - b * kInletRows;
+        d.ni_net[slot] = lrc[t] * lfac[r] - lsur[t];
+    }
+    for (int i = e0 + t + kInletThreads; i < e1; i += kInletThreads) {
+        const int r = d.ie_row[i] - b * kInletRows;
+        d.ni_net[d.ic_slot[i]] = lrc[i - e0] * lfac[r] - lsur[i - e0];
     }
     __shared__ double lds[32];
     double v[2] = {cap_sum, sur_sum};
@@ -272,12 +320,27 @@ __global__ void __launch_bounds__(256) k_inlet_cell(Ctl* __restrict__ ctls, KP<r
 
 // Node volume after the inlet exchange (v1_parity.py:292-298):
 // max(V + sum over the node's inlets of (capture - surcharge), 0), for every node.
-// net0 is ni_net[k0], loaded ahead by the caller (unused when the node has no inlet).
+// net0, net1 are ni_net[k0], ni_net[k0 + 1], loaded ahead by the caller (unused
+// where the node has fewer inlets); later slots are read here.
 template <typename real>
-__device__ __forceinline__ real post_inlet_volume(const DrainDev<real>& d, int k0, int k1, real vol, real net0) {
+__device__ __forceinline__ real post_inlet_volume(const DrainDev<real>& d, int k0, int k1, real vol, real net0,
+                                                  real net1) {
     real acc = real(0);
-    for (int k = k0; k < k1; ++k) acc = acc + (k == k0 ? net0 : d.ni_net[k]);
+    if (k0 < k1) acc = acc + net0;
+    if (k0 + 1 < k1) acc = acc + net1;
+    for (int k = k0 + 2; k < k1; ++k) acc = acc + d.ni_net[k];
     return vmax(vol + acc, real(0));
+}
+
+// The first two inlet-slot nets of a node with inlet slots [k0, k1), for
+// post_inlet_volume (slot 0 where the node has fewer: never used).
+template <typename real>
+struct Nets2 {
+    real n0, n1;
+};
+template <typename real>
+__device__ __forceinline__ Nets2<real> first_nets(const DrainDev<real>& d, int k0, int k1) {
+    return Nets2<real>{d.ni_net[ahead(k0, k1)], d.ni_net[ahead(k0 + 1, k1)]};
 }
 
 template <typename real>
@@ -289,7 +352,7 @@ __device__ __forceinline__ void circular_theta(real depth, real diameter, real& 
     theta = real(2) * acos(x);
 }
 
-// Static data of one link, loaded up front.
+// Data of one link for its momentum update.
 template <typename real>
 struct LinkData {
     real linv, D, len, man, q0, minor, lcap, rw, rh;
@@ -299,8 +362,10 @@ struct LinkData {
 template <typename real>
 __device__ __forceinline__ LinkData<real> load_link(const DrainDev<real>& d, int l) {
     LinkData<real> k;
-    k.linv = d.linv[l]; k.D = d.D[l]; k.len = d.len[l]; k.man = d.man[l]; k.q0 = d.Q[l];
-    k.minor = d.minor[l]; k.lcap = d.lcap[l]; k.rw = d.rw[l]; k.rh = d.rh[l]; k.geom = d.geom[l];
+    const typename V4<real>::type a = d.lk_a[l], b = d.lk_b[l];
+    k.linv = a.x; k.D = a.y; k.len = a.z; k.man = a.w;
+    k.minor = b.x; k.lcap = b.y; k.rw = b.z; k.rh = b.w;
+    k.q0 = d.Q[l]; k.geom = d.geom[l];
     return k;
 }
 
@@ -343,67 +408,91 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const LinkData<r
     return cand * dt;
 }
 
-// Pipe links (one thread per link): the requested transfer Q dt from the
-// post-inlet volumes of the link's two end nodes (v1_parity.py:324-347).
+// Pipe links and donor factors, v1_parity.py:324-352, one block per
+// kNodeBlock nodes.  Every thread computes the requested transfer Q dt of one
+// of the block's link items (from the post-inlet volumes of both end nodes)
+// into LDS; after the block barrier the first kNodeBlock threads, one per node,
+// compute the node's post-inlet volume and donor factor min(1, V / sum of
+// |requested| over the links it donates to, in ascending link order), reading
+// each request from its item.  A link inside the block is one item; a link
+// between two blocks is an item of both, which compute the same value.  The
+// 'from' end stores req[l].  Dynamic LDS: one real per item of the block.
 template <typename real>
-__global__ void __launch_bounds__(256) k_link(const Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    const int mem = d.seg_link.blk_member[blockIdx.x];
-    const int a = d.from[l], b = d.to[l];
+__global__ void __launch_bounds__(kLinkThreads) k_link_node(const Ctl* __restrict__ ctls, KP<real> p,
+                                                           DrainDev<real> d) {
+    constexpr int KB = kRowEntries;
+    extern __shared__ __align__(16) unsigned char fh_lds_dyn[];
+    real* rq = reinterpret_cast<real*>(fh_lds_dyn);
+    const int b = blockIdx.x;
+    const int t = threadIdx.x;
+    const int mem = d.seg_node.blk_member[b];
+    const int mend = d.seg_node.mem_end[mem];
+    const int N = d.N;
+    const int it0 = d.lb_ptr[b], it1 = d.lb_ptr[b + 1];
+    // this thread's first item and its end nodes' data, loaded ahead (threads
+    // past the block's items read item 0, never used)
+    const int q = it0 + t < it1 ? it0 + t : 0;
+    const int l = d.it_l[q], na = d.it_a[q], nb = d.it_b[q];
+    const int a0 = d.it_a0[q], a1 = d.it_a1[q], b0 = d.it_b0[q], b1 = d.it_b1[q];
     const LinkData<real> lk = load_link(d, l);
-    const int a0 = d.lk_a0[l], a1 = d.lk_a1[l], b0 = d.lk_b0[l], b1 = d.lk_b1[l];
-    const real inva = d.lk_inva[l], Asa = d.lk_Asa[l], invb = d.lk_invb[l], Asb = d.lk_Asb[l];
-    const real Va = d.V[a], Vb = d.V[b];
-    const real neta = d.ni_net[ahead(a0, a1)], netb = d.ni_net[ahead(b0, b1)];
-    FH_ISSUE_LOADS();
-    const Ctl* ctl = ctls + mem;
-    if (!ctl->active) return;
-    if (l >= d.seg_link.mem_end[mem]) return;
-    const real dt = (real)ctl->dt_real;
-    const real va = post_inlet_volume(d, a0, a1, Va, neta);
-    const real vb = a == b ? va : post_inlet_volume(d, b0, b1, Vb, netb);
-    d.req[l] = link_request(p, lk, head_of(inva, Asa, va), head_of(invb, Asb, vb), dt);
-}
-
-// Donor factors (one thread per node): the node's post-inlet volume and
-// min(1, V / sum of |requested| over the links it donates to, in ascending link
-// order) -- v1_parity.py:348-352.
-template <typename real>
-__global__ void __launch_bounds__(256) k_node_factor(const Ctl* __restrict__ ctls, DrainDev<real> d) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int mem = d.seg_node.blk_member[blockIdx.x];
+    const real Va = d.V[na], inva = d.inv[na], Asa = d.As[na];
+    const real Vb = d.V[nb], invb = d.inv[nb], Asb = d.As[nb];
+    const Nets2<real> neta = first_nets(d, a0, a1), netb = first_nets(d, b0, b1);
+    // this thread's node (threads < kNodeBlock): inlets, volume, first KB entries
+    const bool node_thread = t < kNodeBlock;
+    const int n = b * kNodeBlock + (node_thread ? t : 0);
     const int i0 = d.ni_ptr[n], i1 = d.ni_ptr[n + 1];
     const int k0 = d.nl_ptr[n], k1 = d.nl_ptr[n + 1];
     const real vol = d.V[n];
-    // the first inlet slot and the first KB link entries, loaded ahead
-    constexpr int KB = kRowEntries;
-    const int N = d.N;
-    const real net0 = d.ni_net[ahead(i0, i1)];
-    int en[KB];
-    real r[KB];
+    const Nets2<real> net = first_nets(d, i0, i1);
+    int en[KB], src[KB];
 #pragma unroll
-    for (int u = 0; u < KB; ++u) en[u] = d.nle[u * N + n];
-#pragma unroll
-    for (int u = 0; u < KB; ++u) r[u] = d.req[en[u] >> 1];
+    for (int u = 0; u < KB; ++u) {
+        en[u] = d.nle[u * N + n];
+        src[u] = d.nls[u * N + n];
+    }
     FH_ISSUE_LOADS();
-    if (!ctls[mem].active) return;
-    if (n >= d.seg_node.mem_end[mem]) return;
-    const real vn = post_inlet_volume(d, i0, i1, vol, net0);
+    const Ctl* ctl = ctls + mem;
+    if (!ctl->active) return;                 // the whole block: one member
+    const real dt = (real)ctl->dt_real;
+    // the node's own post-inlet volume ahead of the barrier
+    const bool live_n = node_thread && n < mend;
+    const real vn = live_n ? post_inlet_volume(d, i0, i1, vol, net.n0, net.n1) : real(0);
+    if (it0 + t < it1) {
+        const real va = post_inlet_volume(d, a0, a1, Va, neta.n0, neta.n1);
+        const real vb = na == nb ? va : post_inlet_volume(d, b0, b1, Vb, netb.n0, netb.n1);
+        rq[t] = link_request(p, lk, head_of(inva, Asa, va), head_of(invb, Asb, vb), dt);
+    }
+    for (int i = it0 + t + kLinkThreads; i < it1; i += kLinkThreads) {
+        const int li = d.it_l[i], ai = d.it_a[i], bi = d.it_b[i];
+        const int ia0 = d.it_a0[i], ia1 = d.it_a1[i], ib0 = d.it_b0[i], ib1 = d.it_b1[i];
+        const Nets2<real> na2 = first_nets(d, ia0, ia1), nb2 = first_nets(d, ib0, ib1);
+        const real va = post_inlet_volume(d, ia0, ia1, d.V[ai], na2.n0, na2.n1);
+        const real vb = ai == bi ? va : post_inlet_volume(d, ib0, ib1, d.V[bi], nb2.n0, nb2.n1);
+        rq[i - it0] = link_request(p, load_link(d, li), head_of(d.inv[ai], d.As[ai], va),
+                                   head_of(d.inv[bi], d.As[bi], vb), dt);
+    }
+    __syncthreads();
+    if (!live_n) return;
     d.Vp[n] = vn;
     real total = real(0);
     for (int kb = k0; kb < k1; kb += KB) {
         if (kb != k0) {
 #pragma unroll
-            for (int u = 0; u < KB; ++u) en[u] = kb + u < k1 ? d.nl_ent[kb + u] : 0;
-#pragma unroll
-            for (int u = 0; u < KB; ++u) r[u] = kb + u < k1 ? d.req[en[u] >> 1] : real(0);
+            for (int u = 0; u < KB; ++u) {
+                if (kb + u >= k1) break;
+                en[u] = d.nl_ent[kb + u];
+                src[u] = d.nl_src[kb + u];
+            }
         }
 #pragma unroll
         for (int u = 0; u < KB; ++u) {
             if (kb + u >= k1) break;
             const bool is_to = en[u] & 1;
-            const bool positive = r[u] >= real(0);
-            if (positive != is_to) total = total + fabs(r[u]);   // donor = from if positive else to
+            const real r = rq[src[u]];
+            if (!is_to) d.req[en[u] >> 1] = r;
+            const bool positive = r >= real(0);
+            if (positive != is_to) total = total + fabs(r);   // donor = from if positive else to
         }
     }
     d.sur_factor[n] = vmin(real(1), vn / vmax(total, real(1.0e-12)));
@@ -413,9 +502,23 @@ __global__ void __launch_bounds__(256) k_node_factor(const Ctl* __restrict__ ctl
 // (reference scatter order: all 'from' links, then all 'to' links), outfall,
 // overflow above storage capacity (weighted entry table, then the node surface
 // cell fallback); writes the link flow for links this node is 'from' of.
+// The blocks after the node blocks apply the inlet cells' depths of this step's
+// inlet exchange (k_inlet leaves the state unchanged for the sibling requests).
 template <typename real>
 __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
+                                                    int node_blocks, typename V4<real>::type* __restrict__ state,
                                                     double* __restrict__ part) {
+    if ((int)blockIdx.x >= node_blocks) {
+        const int bi = blockIdx.x - node_blocks;
+        const int j = bi * blockDim.x + threadIdx.x;
+        const int mi = d.seg_icell.blk_member[bi];
+        const int c = d.ic_cells[j];
+        const real h = d.ic_h[j];
+        FH_ISSUE_LOADS_OF("s"(state));
+        if (!ctls[mi].active) return;
+        if (j < d.seg_icell.mem_end[mi]) state[c].x = h;
+        return;
+    }
     const int b = blockIdx.x;
     const int n = b * blockDim.x + threadIdx.x;
     const int mem = d.seg_node.blk_member[b];

@@ -1150,6 +1150,56 @@ public:
                 ic_capq[k] = capq[i];
                 ic_slot[k] = slot_of_inlet[i];
             }
+            // link work items per block of kNodeBlock nodes: the links leaving a node of
+            // the block (at its 'from' entry), and the links entering it from another
+            // block (at its 'to' entry); each entry reads its link's item
+            const int NB = fh::kNodeBlock;
+            if (nodes.tile != NB) throw ValueError{"node layout tile differs from the node kernels' block"};
+            const int nblk = (NT + NB - 1) / NB;
+            std::vector<int> lb_ptr(nblk + 1, 0), it_l, it_a, it_b, nl_src(nl_ent.size(), 0);
+            std::vector<int> from_item((size_t)std::max(LT, 1), -1);   // block-local item of each link's 'from' entry
+            size_t max_items = 1;
+            for (int bk = 0; bk < nblk; ++bk) {
+                lb_ptr[bk] = (int)it_l.size();
+                const int n0 = bk * NB, n1 = std::min(NT, n0 + NB);
+                for (int n = n0; n < n1; ++n)
+                    for (int k = nl_ptr[n]; k < nl_ptr[n + 1]; ++k) {
+                        const int l = nl_ent[k] >> 1, is_to = nl_ent[k] & 1;
+                        if (!is_to || from[l] / NB != bk) {
+                            const int item = (int)it_l.size() - lb_ptr[bk];
+                            if (!is_to) from_item[l] = item;
+                            nl_src[k] = item;
+                            it_l.push_back(l); it_a.push_back(from[l]); it_b.push_back(to[l]);
+                        }
+                    }
+                for (int n = n0; n < n1; ++n)
+                    for (int k = nl_ptr[n]; k < nl_ptr[n + 1]; ++k) {
+                        const int l = nl_ent[k] >> 1;
+                        if ((nl_ent[k] & 1) && from[l] / NB == bk) nl_src[k] = from_item[l];
+                    }
+                max_items = std::max(max_items, it_l.size() - (size_t)lb_ptr[bk]);
+            }
+            lb_ptr[nblk] = (int)it_l.size();
+            link_node_lds = max_items * sizeof(real);
+            if (link_node_lds > 65536)
+                throw ValueError{"a block of 256 drainage nodes has more links than its LDS holds"};
+            {
+                const size_t ni = it_l.size();
+                std::vector<int> it_a0(ni), it_a1(ni), it_b0(ni), it_b1(ni);
+                for (size_t q = 0; q < ni; ++q) {
+                    it_a0[q] = ni_ptr[it_a[q]]; it_a1[q] = ni_ptr[it_a[q] + 1];
+                    it_b0[q] = ni_ptr[it_b[q]]; it_b1[q] = ni_ptr[it_b[q] + 1];
+                }
+                const int KB = fh::kRowEntries;
+                std::vector<int> nls((size_t)KB * NT, 0);
+                for (int n = 0; n < NT; ++n)
+                    for (int u = 0; u < KB && nl_ptr[n] + u < nl_ptr[n + 1]; ++u)
+                        nls[(size_t)u * NT + n] = nl_src[nl_ptr[n] + u];
+                d_lb_ptr.set(lb_ptr, s); d_it_l.set(it_l, s); d_it_a.set(it_a, s); d_it_b.set(it_b, s);
+                d_it_a0.set(it_a0, s); d_it_a1.set(it_a1, s); d_it_b0.set(it_b0, s); d_it_b1.set(it_b1, s);
+                d_nls.set(nls, s);
+            }
+            d_nl_src.set(nl_src, s);
             // node -> links: 'from' entries then 'to' entries, each ascending, with the far node
             std::vector<int> nl2_ptr(NT + 1, 0), nl2_ent(nl_ent.size()), nl2_oth(nl_ent.size());
             int w = 0;
@@ -1177,11 +1227,9 @@ public:
             // row-aligned copies of each row's first entries
             {
                 const int KB = fh::kRowEntries;
-                std::vector<int> nf_cell(NT, 0), nle((size_t)KB * NT, 0), nl2e((size_t)KB * NT, 0),
-                    nl2o((size_t)KB * NT, 0);
-                std::vector<real> nf_surq(NT, 0), nesw0(NT, 0);
+                std::vector<int> nle((size_t)KB * NT, 0), nl2e((size_t)KB * NT, 0), nl2o((size_t)KB * NT, 0);
+                std::vector<real> nesw0(NT, 0);
                 for (int n = 0; n < NT; ++n) {
-                    if (ni_ptr[n] < ni_ptr[n + 1]) { nf_cell[n] = ni_cell[ni_ptr[n]]; nf_surq[n] = ni_surq[ni_ptr[n]]; }
                     for (int u = 0; u < KB; ++u) {
                         if (nl_ptr[n] + u < nl_ptr[n + 1]) nle[(size_t)u * NT + n] = nl_ent[nl_ptr[n] + u];
                         if (nl2_ptr[n] + u < nl2_ptr[n + 1]) {
@@ -1192,20 +1240,44 @@ public:
                     if (ne_ptr[n] < ne_ptr[n + 1]) nesw0[n] = ne_sw[ne_ptr[n]];
                 }
                 const int CI = icl.total;
-                std::vector<int> icf_node(CI, 0), icf_slot(CI, 0);
-                std::vector<real> icf_capq(CI, 0), icf_inv(CI, 0), icf_As(CI, 1);
-                for (int j = 0; j < CI; ++j) {
-                    if (ic_ptr[j] == ic_ptr[j + 1]) continue;
-                    const int k = ic_ptr[j];
-                    icf_node[j] = ic_node[k]; icf_slot[j] = ic_slot[k]; icf_capq[j] = ic_capq[k];
-                    icf_inv[j] = inv[ic_node[k]]; icf_As[j] = As[ic_node[k]];
+                if (icl.tile != fh::kInletRows) throw ValueError{"inlet-cell layout tile differs from the inlet kernel's"};
+                // per inlet entry: cell, row, node head data, and the node's inlets
+                {
+                    std::vector<int> ie_cell((size_t)nic, 0), ie_row((size_t)nic, 0), ie_s0((size_t)nic, 0),
+                        ie_ns((size_t)nic, 0), ie_sc((size_t)KB * std::max(nic, 1), 0);
+                    std::vector<real> ie_inv((size_t)nic, 0), ie_As((size_t)nic, 1), ie_sq((size_t)KB * std::max(nic, 1), 0);
+                    size_t max_ent = 1;
+                    for (int j = 0; j < CI; ++j) {
+                        if (j % fh::kInletRows == 0) {
+                            const int jb = std::min(CI, j + fh::kInletRows);
+                            max_ent = std::max(max_ent, (size_t)(ic_ptr[jb] - ic_ptr[j]));
+                        }
+                        for (int k = ic_ptr[j]; k < ic_ptr[j + 1]; ++k) {
+                            const int n = ic_node[k];
+                            ie_cell[k] = ic_cells[j];
+                            ie_row[k] = j;
+                            ie_inv[k] = inv[n];
+                            ie_As[k] = As[n];
+                            ie_s0[k] = ni_ptr[n];
+                            ie_ns[k] = ni_ptr[n + 1] - ni_ptr[n];
+                            for (int u = 0; u < KB; ++u) {
+                                const bool has = u < ie_ns[k];
+                                ie_sc[(size_t)u * nic + k] = has ? ni_cell[ni_ptr[n] + u] : ic_cells[j];
+                                ie_sq[(size_t)u * nic + k] = has ? ni_surq[ni_ptr[n] + u] : real(0);
+                            }
+                        }
+                    }
+                    inlet_lds = (2 * max_ent + fh::kInletRows) * sizeof(real);
+                    if (inlet_lds > 65536)
+                        throw ValueError{"a block of 256 inlet cells has more inlets than its LDS holds"};
+                    d_ie_cell.set(ie_cell, s); d_ie_row.set(ie_row, s); d_ie_s0.set(ie_s0, s); d_ie_ns.set(ie_ns, s);
+                    d_ie_sc.set(ie_sc, s); d_ie_inv.set(ie_inv, s); d_ie_As.set(ie_As, s); d_ie_sq.set(ie_sq, s);
+                    d_ic_h.set(std::vector<real>((size_t)CI, 0), s);
                 }
-                std::vector<int> lk_a0(LT, 0), lk_a1(LT, 0), lk_b0(LT, 0), lk_b1(LT, 0);
-                std::vector<real> lk_inva(LT, 0), lk_Asa(LT, 1), lk_invb(LT, 0), lk_Asb(LT, 1);
+                std::vector<typename fh::V4<real>::type> lk_a((size_t)LT), lk_b((size_t)LT);
                 for (int l = 0; l < LT; ++l) {
-                    const int a = from[l], b = to[l];
-                    lk_a0[l] = ni_ptr[a]; lk_a1[l] = ni_ptr[a + 1]; lk_b0[l] = ni_ptr[b]; lk_b1[l] = ni_ptr[b + 1];
-                    lk_inva[l] = inv[a]; lk_Asa[l] = As[a]; lk_invb[l] = inv[b]; lk_Asb[l] = As[b];
+                    lk_a[l].x = linv[l]; lk_a[l].y = D[l]; lk_a[l].z = llen[l]; lk_a[l].w = man[l];
+                    lk_b[l].x = minor[l]; lk_b[l].y = lcap[l]; lk_b[l].z = rw[l]; lk_b[l].w = rh[l];
                 }
                 const int CR = rcl.total;
                 std::vector<int> rcf_node(CR, 0), rcf_fb(CR, 0);
@@ -1214,12 +1286,9 @@ public:
                     if (rc_eptr[j] < rc_eptr[j + 1]) { rcf_node[j] = rc_enode[rc_eptr[j]]; rcf_sw[j] = rc_esw[rc_eptr[j]]; }
                     if (rc_fptr[j] < rc_fptr[j + 1]) rcf_fb[j] = rc_fidx[rc_fptr[j]];
                 }
-                d_nf_cell.set(nf_cell, s); d_nf_surq.set(nf_surq, s); d_nle.set(nle, s); d_nl2e.set(nl2e, s);
+                d_nle.set(nle, s); d_nl2e.set(nl2e, s);
                 d_nl2o.set(nl2o, s); d_nesw0.set(nesw0, s);
-                d_icf_node.set(icf_node, s); d_icf_slot.set(icf_slot, s); d_icf_capq.set(icf_capq, s);
-                d_icf_inv.set(icf_inv, s); d_icf_As.set(icf_As, s);
-                d_lk_a0.set(lk_a0, s); d_lk_a1.set(lk_a1, s); d_lk_b0.set(lk_b0, s); d_lk_b1.set(lk_b1, s);
-                d_lk_inva.set(lk_inva, s); d_lk_Asa.set(lk_Asa, s); d_lk_invb.set(lk_invb, s); d_lk_Asb.set(lk_Asb, s);
+                d_lk_a.set(lk_a, s); d_lk_b.set(lk_b, s);
                 d_rcf_node.set(rcf_node, s); d_rcf_sw.set(rcf_sw, s); d_rcf_fb.set(rcf_fb, s);
                 std::vector<real> ic_area(CI, 1), rc_area(CR, 1);
                 for (int j = 0; j < CI; ++j) ic_area[j] = area_of_cell[(size_t)ic_cells[j]];
@@ -1273,12 +1342,14 @@ public:
         d.ne_ptr = d_ne_ptr.p; d.ne_sw = d_ne_sw.p;
         d.ic_cells = d_ic_cells.p; d.ic_ptr = d_ic_ptr.p; d.ic_node = d_ic_node.p; d.ic_slot = d_ic_slot.p;
         d.ic_capq = d_ic_capq.p;
-        d.nf_cell = d_nf_cell.p; d.nf_surq = d_nf_surq.p; d.nle = d_nle.p; d.nl2e = d_nl2e.p; d.nl2o = d_nl2o.p;
+        d.nle = d_nle.p; d.nl2e = d_nl2e.p; d.nl2o = d_nl2o.p;
+        d.lb_ptr = d_lb_ptr.p; d.it_l = d_it_l.p; d.it_a = d_it_a.p; d.it_b = d_it_b.p;
+        d.it_a0 = d_it_a0.p; d.it_a1 = d_it_a1.p; d.it_b0 = d_it_b0.p; d.it_b1 = d_it_b1.p;
+        d.nl_src = d_nl_src.p; d.nls = d_nls.p;
+        d.ie_cell = d_ie_cell.p; d.ie_row = d_ie_row.p; d.ie_s0 = d_ie_s0.p; d.ie_ns = d_ie_ns.p;
+        d.ie_sc = d_ie_sc.p; d.ie_inv = d_ie_inv.p; d.ie_As = d_ie_As.p; d.ie_sq = d_ie_sq.p;
+        d.ic_h = d_ic_h.p; d.lk_a = d_lk_a.p; d.lk_b = d_lk_b.p;
         d.nesw0 = d_nesw0.p;
-        d.icf_node = d_icf_node.p; d.icf_slot = d_icf_slot.p; d.icf_capq = d_icf_capq.p; d.icf_inv = d_icf_inv.p;
-        d.icf_As = d_icf_As.p;
-        d.lk_a0 = d_lk_a0.p; d.lk_a1 = d_lk_a1.p; d.lk_b0 = d_lk_b0.p; d.lk_b1 = d_lk_b1.p;
-        d.lk_inva = d_lk_inva.p; d.lk_Asa = d_lk_Asa.p; d.lk_invb = d_lk_invb.p; d.lk_Asb = d_lk_Asb.p;
         d.rcf_node = d_rcf_node.p; d.rcf_sw = d_rcf_sw.p; d.rcf_fb = d_rcf_fb.p;
         d.ic_area = d_ic_area.p; d.rc_area = d_rc_area.p;
         d.from = d_from.p; d.to = d_to.p; d.geom = d_geom.p; d.len = d_llen.p; d.man = d_man.p; d.lcap = d_lcap.p;
@@ -1305,11 +1376,15 @@ public:
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     template <typename... KA, typename... A>
     void go(void (*kernel)(KA...), unsigned grid, unsigned block, A... args) {
+        go_lds(kernel, grid, block, 0, args...);
+    }
+    template <typename... KA, typename... A>
+    void go_lds(void (*kernel)(KA...), unsigned grid, unsigned block, size_t lds, A... args) {
         if (ev_start)
-            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, ev_start, ev_stop, 0u,
+            hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, ev_start, ev_stop, 0u,
                                   static_cast<KA>(args)...);
         else
-            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, stream, static_cast<KA>(args)...);
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, static_cast<KA>(args)...);
     }
 
     // CFL minimum over every cell of each member into cfl_min (stable_timestep)
@@ -1353,23 +1428,17 @@ public:
                 go(fh::k_external<real>, xcl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_ext.p);
             });
         if (nodes.blocks == 0) return;
-        if (icl.blocks > 0) {
-            timed("inlet_node_factor", (double)d.N * (8 + 3 * rb) + (double)d.I * (12 + 5 * rb), [&] {
-                go(fh::k_inlet_node_factor<real>, nodes.blocks, 256, ctl, kp, d, d_state.p);
+        if (icl.blocks > 0)
+            timed("inlet", (double)d.Ci * (12 + 4 * rb) + (double)d.I * (8 + 5 * rb), [&] {
+                go_lds(fh::k_inlet<real>, icl.blocks, fh::kInletThreads, inlet_lds, ctl, kp, d, d_state.p,
+                       d_part_inlet.p);
             });
-            timed("inlet_cell", (double)d.Ci * (12 + 4 * rb) + (double)d.I * (8 + 5 * rb), [&] {
-                go(fh::k_inlet_cell<real>, icl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_inlet.p);
-            });
-        }
-        if (links.blocks > 0)
-            timed("link", (double)d.L * (12 + 12 * rb) + (double)d.N * 4 * rb, [&] {
-                go(fh::k_link<real>, links.blocks, 256, ctl, kp, d);
-            });
-        timed("node_factor", (double)d.N * (8 + 3 * rb) + (double)d.L * (8 + rb) + (double)d.I * (4 + 2 * rb), [&] {
-            go(fh::k_node_factor<real>, nodes.blocks, 256, ctl, d);
+        timed("link_node", (double)d.N * (8 + 4 * rb) + (double)d.L * (8 + 11 * rb), [&] {
+            go_lds(fh::k_link_node<real>, nodes.blocks, fh::kLinkThreads, link_node_lds, ctl, kp, d);
         });
         timed("node_update", (double)d.N * (16 + 14 * rb) + (double)d.L * (16 + 3 * rb), [&] {
-            go(fh::k_node_update<real>, nodes.blocks, 256, ctl, kp, d, d_part_node.p);
+            go(fh::k_node_update<real>, nodes.blocks + icl.blocks, 256, ctl, kp, d, nodes.blocks, d_state.p,
+               d_part_node.p);
         });
         if (rcl.blocks > 0)
             timed("return", (double)d.Cr * (16 + 3 * rb) + (double)d.N * 2 * rb, [&] {
@@ -1637,11 +1706,15 @@ public:
     DevArray<real> d_sw;
     DevArray<int> d_rc_cells, d_rc_eptr, d_rc_enode, d_rc_fptr, d_rc_fidx, d_rc_rptr, d_rc_ridx;
     DevArray<real> d_rc_esw;
-    DevArray<int> d_nf_cell, d_nle, d_nl2e, d_nl2o, d_icf_node, d_icf_slot, d_lk_a0, d_lk_a1, d_lk_b0, d_lk_b1,
-        d_rcf_node, d_rcf_fb;
+    DevArray<int> d_nle, d_nl2e, d_nl2o, d_rcf_node, d_rcf_fb;
+    DevArray<int> d_lb_ptr, d_it_l, d_it_a, d_it_b, d_it_a0, d_it_a1, d_it_b0, d_it_b1, d_nl_src, d_nls;
+    DevArray<int> d_ie_cell, d_ie_row, d_ie_s0, d_ie_ns, d_ie_sc;
+    DevArray<real> d_ie_inv, d_ie_As, d_ie_sq, d_ic_h;
+    size_t link_node_lds = sizeof(real);   // dynamic LDS of k_link_node
+    size_t inlet_lds = sizeof(real);       // dynamic LDS of k_inlet
+    DevArray<V> d_lk_a, d_lk_b;
     DevArray<real> d_ic_area, d_rc_area;
-    DevArray<real> d_nf_surq, d_nesw0, d_icf_capq, d_icf_inv, d_icf_As, d_lk_inva, d_lk_Asa, d_lk_invb, d_lk_Asb,
-        d_rcf_sw;
+    DevArray<real> d_nesw0, d_rcf_sw;
     DevArray<real> d_rS, d_rcc, d_rrc, d_rstop, d_rstore;
     DevArray<int> d_rcv_qbase, d_rcv_stride, d_xc_cells, d_xc_ptr, d_xc_idx;
     fh::DrainDev<real> dd{};
