@@ -21,6 +21,14 @@ template <typename real>
 struct DrainDev {
     int N, I, Ci, L, S, Cr, R, Rcv, Cx;   // padded totals of the batch
     Seg seg_node, seg_link, seg_icell, seg_ret, seg_ext;
+    // Return rows k_return handles (rx_rows, blocks seg_rx): those with entry
+    // returns, recharge, or several fallback nodes.  Every other row has exactly
+    // one fallback node and nothing else; that node's thread in k_node_update
+    // applies it: nr_cell [N] (-1: none) and nr_area [N] its cell and area.
+    Seg seg_rx;
+    const int* rx_rows;
+    const int* nr_cell;
+    const real* nr_area;
     // nodes
     const real *inv, *As, *dmax, *o_area, *o_coef, *o_tw, *o_cap;
     const int* nsc;
@@ -417,11 +425,26 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const LinkData<r
 // each request from its item.  A link inside the block is one item; a link
 // between two blocks is an item of both, which compute the same value.  The
 // 'from' end stores req[l].  Dynamic LDS: one real per item of the block.
+// The blocks after the node blocks (kLinkThreads inlet cells each) apply the
+// inlet cells' new depths.
 template <typename real>
 __global__ void __launch_bounds__(kLinkThreads) k_link_node(const Ctl* __restrict__ ctls, KP<real> p,
-                                                           DrainDev<real> d) {
+        DrainDev<real> d, int node_blocks, typename V4<real>::type* __restrict__ state) {
     constexpr int KB = kRowEntries;
     extern __shared__ __align__(16) unsigned char fh_lds_dyn[];
+    if ((int)blockIdx.x >= node_blocks) {
+        // the inlet cells' depths of this step's inlet exchange (k_inlet leaves
+        // the state unchanged for the sibling requests)
+        const int j = ((int)blockIdx.x - node_blocks) * (int)blockDim.x + (int)threadIdx.x;
+        if (j >= d.Ci) return;
+        const int mi = d.seg_icell.blk_member[j / kInletRows];
+        const int c = d.ic_cells[j];
+        const real h = d.ic_h[j];
+        FH_ISSUE_LOADS_OF("s"(state));
+        if (!ctls[mi].active) return;
+        if (j < d.seg_icell.mem_end[mi]) state[c].x = h;
+        return;
+    }
     real* rq = reinterpret_cast<real*>(fh_lds_dyn);
     const int b = blockIdx.x;
     const int t = threadIdx.x;
@@ -502,23 +525,12 @@ __global__ void __launch_bounds__(kLinkThreads) k_link_node(const Ctl* __restric
 // (reference scatter order: all 'from' links, then all 'to' links), outfall,
 // overflow above storage capacity (weighted entry table, then the node surface
 // cell fallback); writes the link flow for links this node is 'from' of.
-// The blocks after the node blocks apply the inlet cells' depths of this step's
-// inlet exchange (k_inlet leaves the state unchanged for the sibling requests).
+// A node whose fallback volume is the only return its surface cell receives
+// (nr_cell >= 0) applies it here, as k_return would: h + (0 + fb / A).
 template <typename real>
 __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
-                                                    int node_blocks, typename V4<real>::type* __restrict__ state,
+                                                    typename V4<real>::type* __restrict__ state,
                                                     double* __restrict__ part) {
-    if ((int)blockIdx.x >= node_blocks) {
-        const int bi = blockIdx.x - node_blocks;
-        const int j = bi * blockDim.x + threadIdx.x;
-        const int mi = d.seg_icell.blk_member[bi];
-        const int c = d.ic_cells[j];
-        const real h = d.ic_h[j];
-        FH_ISSUE_LOADS_OF("s"(state));
-        if (!ctls[mi].active) return;
-        if (j < d.seg_icell.mem_end[mi]) state[c].x = h;
-        return;
-    }
     const int b = blockIdx.x;
     const int n = b * blockDim.x + threadIdx.x;
     const int mem = d.seg_node.blk_member[b];
@@ -545,9 +557,12 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         fo[u] = d.sur_factor[ot[u]];
     }
     const real sw0 = d.nesw0[n];
+    const int rcell = d.nr_cell[n];
+    const real rA = d.nr_area[n];
+    const real rh = state[rcell >= 0 ? rcell : 0].x;
     SlotAcc<3> acc3;
     acc3.prefetch(part, b);
-    FH_ISSUE_LOADS_OF("s"(part));
+    FH_ISSUE_LOADS_OF("s"(part), "s"(state));
     Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
     double out_sum = 0.0, total_sum = 0.0, unret_sum = 0.0;
@@ -613,6 +628,11 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         d.overflow[n] = over;
         d.wsum[n] = wsum;
         d.fallback[n] = fb;
+        if (rcell >= 0) {
+            real acc = zero;
+            acc = acc + fb / rA;
+            state[rcell].x = rh + acc;
+        }
         out_sum = (double)outf;
         total_sum = (double)(returned + fb);
         unret_sum = (double)vmax(vol - capacity, zero);
@@ -623,14 +643,16 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
     acc3.commit(part, b, v);
 }
 
-// Return pass (one thread per receiving cell): weighted entry returns, node
-// fallback returns, then recharge capture/loss (v1_parity.py:415-427).
+// Return pass (one thread per receiving cell of rx_rows): weighted entry
+// returns, node fallback returns, then recharge capture/loss
+// (v1_parity.py:371-404, 415-427).
 template <typename real>
 __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
         typename V4<real>::type* __restrict__ state, const real* __restrict__ area, double* __restrict__ part) {
     const int b = blockIdx.x;
-    const int j = b * blockDim.x + threadIdx.x;
-    const int mem = d.seg_ret.blk_member[b];
+    const int x = b * blockDim.x + threadIdx.x;
+    const int mem = d.seg_rx.blk_member[b];
+    const int j = d.rx_rows[x];
     const real zero = real(0);
     const int c = d.rc_cells[j];
     const int ke0 = d.rc_eptr[j], ke1 = d.rc_eptr[j + 1];
@@ -648,7 +670,7 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
     Ctl* ctl = ctls + mem;
     if (!ctl->active) return;
     double cap_sum = 0.0, loss_sum = 0.0;
-    if (j < d.seg_ret.mem_end[mem]) {
+    if (x < d.seg_rx.mem_end[mem]) {
         const real dt = (real)ctl->dt_real;
         real h = h0;
         if (d.S > 0) {

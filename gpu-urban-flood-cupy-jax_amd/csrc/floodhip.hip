@@ -1294,6 +1294,29 @@ public:
                 for (int j = 0; j < CI; ++j) ic_area[j] = area_of_cell[(size_t)ic_cells[j]];
                 for (int j = 0; j < CR; ++j) rc_area[j] = area_of_cell[(size_t)rc_cells[j]];
                 d_ic_area.set(ic_area, s); d_rc_area.set(rc_area, s);
+                // return rows with one fallback node and nothing else go to that node's
+                // thread in the node update; the others stay with the return kernel
+                std::vector<int> nr_cell(NT, -1);
+                std::vector<real> nr_area(NT, 1);
+                std::vector<std::vector<int>> rx(M);
+                for (int i = 0; i < M; ++i)
+                    for (int j = rcl.base[i]; j < rcl.base[i] + rcl.count[i]; ++j) {
+                        const bool simple = rc_eptr[j] == rc_eptr[j + 1] && rc_fptr[j + 1] - rc_fptr[j] == 1 &&
+                                            rc_rptr[j] == rc_rptr[j + 1];
+                        if (simple) {
+                            const int n = rc_fidx[rc_fptr[j]];
+                            nr_cell[n] = rc_cells[j];
+                            nr_area[n] = rc_area[j];
+                        } else {
+                            rx[i].push_back(j);
+                        }
+                    }
+                rxl.build(counts_of(rx), 256);
+                std::vector<int> rx_rows((size_t)rxl.total, 0);
+                for (int i = 0; i < M; ++i)
+                    for (size_t k = 0; k < rx[i].size(); ++k) rx_rows[(size_t)rxl.base[i] + k] = rx[i][k];
+                d_rx_rows.set(rx_rows, s); d_nr_cell.set(nr_cell, s); d_nr_area.set(nr_area, s);
+                seg_rx.set(rxl, order, s);
             }
             d_ni_cell.set(ni_cell, s); d_ni_surq.set(ni_surq, s);
             std::vector<real> zS((size_t)nslots, 0);
@@ -1322,7 +1345,7 @@ public:
         seg_rcl.set(rcl, order, s); seg_xcl.set(xcl, order, s);
         d_part_inlet.set(std::vector<double>(2 * (size_t)std::max(icl.blocks, 1), 0.0), s);
         d_part_node.set(std::vector<double>(3 * (size_t)std::max(nodes.blocks, 1), 0.0), s);
-        d_part_ret.set(std::vector<double>(2 * (size_t)std::max(rcl.blocks, 1), 0.0), s);
+        d_part_ret.set(std::vector<double>(2 * (size_t)std::max(rxl.blocks, 1), 0.0), s);
         d_part_ext.set(std::vector<double>((size_t)std::max(xcl.blocks, 1), 0.0), s);
         n_receivers = Xt;
         // device view
@@ -1332,6 +1355,7 @@ public:
         d.Cx = xcl.total;
         d.seg_node = seg_nodes.seg(); d.seg_link = seg_links.seg(); d.seg_icell = seg_icl.seg();
         d.seg_ret = seg_rcl.seg(); d.seg_ext = seg_xcl.seg();
+        d.seg_rx = seg_rx.seg(); d.rx_rows = d_rx_rows.p; d.nr_cell = d_nr_cell.p; d.nr_area = d_nr_area.p;
         d.inv = d_inv.p; d.As = d_As.p; d.dmax = d_dmax.p; d.o_area = d_oa.p; d.o_coef = d_oc.p; d.o_tw = d_otw.p;
         d.o_cap = d_ocap.p; d.nsc = d_nsc.p; d.V = d_V.p; d.Vp = d_Vp.p; d.sur_factor = d_sfac.p;
         d.overflow = d_over.p;
@@ -1434,15 +1458,15 @@ public:
                        d_part_inlet.p);
             });
         timed("link_node", (double)d.N * (8 + 4 * rb) + (double)d.L * (8 + 11 * rb), [&] {
-            go_lds(fh::k_link_node<real>, nodes.blocks, fh::kLinkThreads, link_node_lds, ctl, kp, d);
+            go_lds(fh::k_link_node<real>, nodes.blocks + (icl.total + fh::kLinkThreads - 1) / fh::kLinkThreads,
+                   fh::kLinkThreads, link_node_lds, ctl, kp, d, nodes.blocks, d_state.p);
         });
         timed("node_update", (double)d.N * (16 + 14 * rb) + (double)d.L * (16 + 3 * rb), [&] {
-            go(fh::k_node_update<real>, nodes.blocks + icl.blocks, 256, ctl, kp, d, nodes.blocks, d_state.p,
-               d_part_node.p);
+            go(fh::k_node_update<real>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
         });
-        if (rcl.blocks > 0)
+        if (rxl.blocks > 0)
             timed("return", (double)d.Cr * (16 + 3 * rb) + (double)d.N * 2 * rb, [&] {
-                go(fh::k_return<real>, rcl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_ret.p);
+                go(fh::k_return<real>, rxl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_ret.p);
             });
     }
 
@@ -1507,7 +1531,7 @@ public:
                                {fh::L_RAIN, fh::L_INFIL, fh::L_BIN, fh::L_BOUT}};
         group(1, d_part_inlet.p, seg_icl, 2, fh::L_CAPTURE, fh::L_SURCHARGE, 0);
         group(2, d_part_node.p, seg_nodes, 3, fh::L_OUTFALL, fh::L_OVERFLOW, fh::L_UNRETURNED);
-        group(3, d_part_ret.p, seg_rcl, 2, fh::L_RCAPTURE, fh::L_RLOSS, 0);
+        group(3, d_part_ret.p, seg_rx, 2, fh::L_RCAPTURE, fh::L_RLOSS, 0);
         group(4, d_part_ext.p, seg_xcl, 1, fh::L_EXTERNAL, 0, 0);
         f.ngroups = 5;
         fh::k_fold<<<M, 256, 0, stream>>>(ctl, f);
@@ -1682,8 +1706,10 @@ public:
     int par = 0;                // parity of the clock slot the next step runs on
     bool packed = false, dirty = true, any_drainage = false;
     int n_receivers = 0;
-    Layout cells, edges, nodes, links, icl, rcl, xcl, begin_l, orphans;
-    DevSeg seg_cells, seg_edges, seg_begin, seg_nodes, seg_links, seg_icl, seg_rcl, seg_xcl, seg_orph;
+    Layout cells, edges, nodes, links, icl, rcl, xcl, begin_l, orphans, rxl;
+    DevSeg seg_cells, seg_edges, seg_begin, seg_nodes, seg_links, seg_icl, seg_rcl, seg_xcl, seg_orph, seg_rx;
+    DevArray<int> d_rx_rows, d_nr_cell;
+    DevArray<real> d_nr_area;
     DevArray<int> d_cell0, d_cell1, d_rain_off, d_rain_n, d_ext_off, d_ext_n, d_ocells;
     DevArray<int> d_el, d_er, d_ce0, d_ce1, d_ce2;
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
