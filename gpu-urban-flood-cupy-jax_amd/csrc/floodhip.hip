@@ -822,7 +822,7 @@ public:
         icl.build(counts_of(icells), 256);
         rcl.build(counts_of(rcells), 256);
         xcl.build(counts_of(xcells), 256);
-        // k_begin blocks: proportional to the member's cells
+        // k_cfl blocks: proportional to the member's cells
         {
             std::vector<int64_t> nb(M);
             for (int i = 0; i < M; ++i) nb[i] = std::max<int64_t>(1, std::min<int64_t>(128, (mem[order[i]].C + 8191) / 8192));
