@@ -160,3 +160,25 @@ def oracle_run(inputs, duration_s, snapshot_interval_s=3600.0, max_steps_per_hou
     report = {"max_depth_m": float(np.max(depth_a)), **{f"{k}_m3": v for k, v in ledger.items()},
               "mass_residual_ratio": (lhs - rhs) / max(abs(lhs), 1.0), "step_count": clock.global_step}
     return {"snapshot_times_s": np.asarray(times, np.float32), "depth_snapshots_m": depth_a, "report": report}
+
+
+GOLDEN_DIR = __import__("pathlib").Path(__file__).resolve().parent / "golden"
+_GROUP_TYPES = {"surface": types.SurfaceTopology, "coupling": types.CouplingTopology, "pipe": types.PipeTopology,
+                "recharge": types.RechargeTopology, "initial_state": types.ParityState,
+                "forcing": types.ParityForcing}
+
+
+def load_golden(name):
+    """A fixture written by tests/golden/make_golden.py: (LoadedParityInputs, schedule
+    dict with dt/rain/q rows, expected ParityState, expected ledger dict, dtype)."""
+    with np.load(GOLDEN_DIR / f"{name}.npz", allow_pickle=False) as z:
+        a = {k: z[k] for k in z.files}
+    groups = {}
+    for g, cls in _GROUP_TYPES.items():
+        groups[g] = cls(*[(a[f"{g}.{f}"].item() if a[f"{g}.{f}"].ndim == 0 else a[f"{g}.{f}"])
+                          for f in cls._fields])
+    inp = types.LoadedParityInputs(groups["surface"], groups["coupling"], groups["pipe"], groups["recharge"],
+                                   groups["initial_state"], groups["forcing"], {}, {})
+    expected = types.ParityState(*[a[f"out.{f}"] for f in types.ParityState._fields])
+    ledger = {k: float(a[f"ledger.{k}"]) for k in orc.LEDGER_FIELDS}
+    return inp, {"dt": a["dt"], "rain": a["rain"], "q": a["q"]}, expected, ledger, np.dtype(str(a["dtype"]))
