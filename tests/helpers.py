@@ -182,3 +182,17 @@ def load_golden(name):
     expected = types.ParityState(*[a[f"out.{f}"] for f in types.ParityState._fields])
     ledger = {k: float(a[f"ledger.{k}"]) for k in orc.LEDGER_FIELDS}
     return inp, {"dt": a["dt"], "rain": a["rain"], "q": a["q"]}, expected, ledger, np.dtype(str(a["dtype"]))
+
+
+def strip_network(inp, links=True, inlets=True):
+    """The scenario with its pipe links and/or its inlets removed (nodes, outfalls,
+    overflow returns and recharge stay): the degenerate networks of a drainage
+    model that has storage nodes without pipes or pipes without inlets."""
+    coupling, pipe, state = inp.coupling, inp.pipe, inp.initial_state
+    if links:
+        pipe = pipe._replace(**{f: np.asarray(v)[:0] for f, v in pipe._asdict().items() if f.startswith("link_")})
+        state = state._replace(link_flow=np.zeros(0, np.float32))
+    if inlets:
+        coupling = coupling._replace(**{f: np.asarray(v)[:0] for f, v in coupling._asdict().items()
+                                        if f.startswith("inlet_")})
+    return inp._replace(coupling=coupling, pipe=pipe, initial_state=state)

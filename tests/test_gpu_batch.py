@@ -8,7 +8,7 @@ import pytest
 
 from floodhip import synth
 from floodhip.solver import HipEnsemble, HipParitySolver
-from helpers import small_network_scenario
+from helpers import small_network_scenario, strip_network
 
 pytestmark = pytest.mark.gpu
 
@@ -19,7 +19,8 @@ def _members():
     a = a._replace(initial_state=a.initial_state._replace(h=h0))
     b = small_network_scenario(seed=5, h_init=0.3)
     c = synth.make_catchment(6000, seed=synth.BASE_SEED + 77, rain_hours=1, rain_mm_h=[90.0])
-    return [a, b, c]
+    d = strip_network(small_network_scenario(seed=9, h_init=0.3), links=True, inlets=False)   # nodes, no pipes
+    return [a, b, c, d]
 
 
 def _plain(inp, **kw):
