@@ -30,12 +30,17 @@ struct DrainDev {
     const int* nr_cell;
     const real* nr_area;
     // nodes
-    const real *inv, *As, *dmax, *o_area, *o_coef, *o_tw, *o_cap;
-    const int* nsc;
+    const real *inv, *As, *dmax;
+    // [N] bit 0: the overflow left after the entry table falls back to the
+    // node's surface cell (no valid entry rows, surface cell >= 0); bits 1..:
+    // outfall slot + 1, or 0 where the outfall term is exactly zero (coefficient
+    // x area == 0 with finite tailwater and capacity)
+    const int* nflag;
+    const typename V4<real>::type* outfall;   // [slots] {tailwater, coefficient, area, capacity}
     real* V;
     real* Vp;               // node volume after the inlet exchange of this step
     real *sur_factor, *overflow, *fallback, *wsum;
-    const int* explicit_;  // node has valid entry-table rows
+    int store_returns;      // k_return runs: k_node_update stores overflow, weight sum, fallback
     // node -> inlets, ascending inlet ("slots"): the inlet's cell and surcharge
     // limit, and per step the scaled surcharge and net exchange (capture - surcharge)
     const int *ni_ptr, *ni_cell;
@@ -536,11 +541,11 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
     const int mem = d.seg_node.blk_member[b];
     const real zero = real(0);
     const int k0 = d.nl2_ptr[n], k1 = d.nl2_ptr[n + 1];
-    const int e0 = d.ne_ptr[n], e1 = d.ne_ptr[n + 1];
+    const bool entries = d.S > 0;
+    const int e0 = entries ? d.ne_ptr[n] : 0, e1 = entries ? d.ne_ptr[n + 1] : 0;
     const real vp = d.Vp[n], f_self = d.sur_factor[n];
-    const real o_tw = d.o_tw[n], o_coef = d.o_coef[n], o_area = d.o_area[n], o_cap = d.o_cap[n];
     const real inv = d.inv[n], As = d.As[n], dmax = d.dmax[n];
-    const int expl = d.explicit_[n], nsc = d.nsc[n];
+    const int flag = d.nflag[n];
     // the first KB link entries and the first entry weight, loaded ahead
     constexpr int KB = kRowEntries;
     const int N = d.N;
@@ -556,7 +561,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         r[u] = d.req[en[u] >> 1];
         fo[u] = d.sur_factor[ot[u]];
     }
-    const real sw0 = d.nesw0[n];
+    const real sw0 = entries ? d.nesw0[n] : zero;
     const int rcell = d.nr_cell[n];
     const real rA = d.nr_area[n];
     const real rh = state[rcell >= 0 ? rcell : 0].x;
@@ -599,13 +604,18 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
             }
             vol = vmax(vol + delta, zero);
         }
-        // outfall (v1_parity.py:361-367)
-        const real hd = head_of(inv, As, vol) - o_tw;
-        const real sgn = hd > zero ? real(1) : (hd < zero ? real(-1) : zero);
-        real oq = o_coef * o_area * sqrt(p.two_g * fabs(hd)) * sgn;
-        oq = vmin(oq, vmax(o_cap, zero));
-        const real outf = vmin(vol, vmax(oq, zero) * dt);
-        vol = vmax(vol - outf, zero);
+        // outfall (v1_parity.py:361-367); a node without an outfall slot
+        // discharges exactly zero
+        real outf = zero;
+        if (flag >> 1) {
+            const typename V4<real>::type o = d.outfall[(flag >> 1) - 1];
+            const real hd = head_of(inv, As, vol) - o.x;
+            const real sgn = hd > zero ? real(1) : (hd < zero ? real(-1) : zero);
+            real oq = o.y * o.z * sqrt(p.two_g * fabs(hd)) * sgn;
+            oq = vmin(oq, vmax(o.w, zero));
+            outf = vmin(vol, vmax(oq, zero) * dt);
+            vol = vmax(vol - outf, zero);
+        }
         // overflow (v1_parity.py:371-404)
         const real capacity = As * vmax(dmax, zero);
         const real over = vmax(vol - capacity, zero);
@@ -622,13 +632,16 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         }
         vol = vol - returned;
         const real residual = vmax(vol - capacity, zero);
-        const real fb = (!expl && nsc >= 0) ? residual : zero;
+        const real fb = (flag & 1) ? residual : zero;
         vol = vol - fb;
         d.V[n] = vol;
-        d.overflow[n] = over;
-        d.wsum[n] = wsum;
-        d.fallback[n] = fb;
-        if (rcell >= 0) {
+        if (d.store_returns) {
+            d.overflow[n] = over;
+            d.wsum[n] = wsum;
+            d.fallback[n] = fb;
+        }
+        // h + (0 + fb / A) leaves h's bits unchanged when fb is +0, unless h is -0
+        if (rcell >= 0 && (fb != zero || (rh == zero && signbit(rh)))) {
             real acc = zero;
             acc = acc + fb / rA;
             state[rcell].x = rh + acc;

@@ -1125,8 +1125,28 @@ public:
             build_csr(rrow, rcl.total, rc_rptr, rc_ridx);
             build_csr(xrow, xcl.total, xc_ptr, xc_idx);
         }
-        d_inv.set(inv, s); d_As.set(As, s); d_dmax.set(dmax, s); d_oa.set(oa, s); d_oc.set(oc, s); d_otw.set(otw, s);
-        d_ocap.set(ocap, s); d_V.set(V, s); d_nsc.set(nsc, s); d_expl.set(expl, s);
+        d_inv.set(inv, s); d_As.set(As, s); d_dmax.set(dmax, s); d_V.set(V, s);
+        {
+            // node flags and the outfall slots: a node whose coefficient x area is
+            // zero (with finite tailwater and capacity) discharges exactly zero
+            std::vector<int> nflag(NT, 0);
+            typedef typename fh::V4<real>::type V4r;
+            std::vector<V4r> otab;
+            for (int n = 0; n < NT; ++n) {
+                const real prod = oc[n] * oa[n];
+                const bool none = prod == real(0) && std::isfinite(otw[n]) && std::isfinite(ocap[n]);
+                int f = (!expl[n] && nsc[n] >= 0) ? 1 : 0;
+                if (!none) {
+                    V4r o;
+                    o.x = otw[n]; o.y = oc[n]; o.z = oa[n]; o.w = ocap[n];
+                    otab.push_back(o);
+                    f |= (int)otab.size() << 1;
+                }
+                nflag[n] = f;
+            }
+            d_nflag.set(nflag, s);
+            d_outfall.set(otab, s);
+        }
         std::vector<real> zN(NT, 0);
         d_Vp.set(zN, s);
         d_sfac.set(zN, s); d_over.set(zN, s); d_fb.set(zN, s); d_wsum.set(zN, s);
@@ -1356,10 +1376,10 @@ public:
         d.seg_node = seg_nodes.seg(); d.seg_link = seg_links.seg(); d.seg_icell = seg_icl.seg();
         d.seg_ret = seg_rcl.seg(); d.seg_ext = seg_xcl.seg();
         d.seg_rx = seg_rx.seg(); d.rx_rows = d_rx_rows.p; d.nr_cell = d_nr_cell.p; d.nr_area = d_nr_area.p;
-        d.inv = d_inv.p; d.As = d_As.p; d.dmax = d_dmax.p; d.o_area = d_oa.p; d.o_coef = d_oc.p; d.o_tw = d_otw.p;
-        d.o_cap = d_ocap.p; d.nsc = d_nsc.p; d.V = d_V.p; d.Vp = d_Vp.p; d.sur_factor = d_sfac.p;
+        d.inv = d_inv.p; d.As = d_As.p; d.dmax = d_dmax.p; d.nflag = d_nflag.p; d.outfall = d_outfall.p;
+        d.V = d_V.p; d.Vp = d_Vp.p; d.sur_factor = d_sfac.p;
         d.overflow = d_over.p;
-        d.fallback = d_fb.p; d.wsum = d_wsum.p; d.explicit_ = d_expl.p;
+        d.fallback = d_fb.p; d.wsum = d_wsum.p; d.store_returns = rxl.blocks > 0 ? 1 : 0;
         d.ni_ptr = d_ni_ptr.p; d.ni_cell = d_ni_cell.p; d.ni_surq = d_ni_surq.p; d.ni_sur = d_ni_sur.p;
         d.ni_net = d_ni_net.p; d.nl_ptr = d_nl_ptr.p; d.nl_ent = d_nl_ent.p;
         d.nl2_ptr = d_nl2_ptr.p; d.nl2_ent = d_nl2_ent.p; d.nl2_oth = d_nl2_oth.p;
@@ -1722,8 +1742,9 @@ public:
         d_part_ext;
     DevArray<float> d_rain_s, d_rain_e, d_rain_r, d_ext_t, d_ext_q, d_ext_ov;
     // drainage
-    DevArray<real> d_inv, d_As, d_dmax, d_oa, d_oc, d_otw, d_ocap, d_V, d_Vp, d_sfac, d_over, d_fb, d_wsum;
-    DevArray<int> d_nsc, d_expl, d_ni_ptr, d_ni_cell, d_nl_ptr, d_nl_ent, d_nl2_ptr, d_nl2_ent, d_nl2_oth, d_ne_ptr;
+    DevArray<real> d_inv, d_As, d_dmax, d_V, d_Vp, d_sfac, d_over, d_fb, d_wsum;
+    DevArray<V> d_outfall;
+    DevArray<int> d_nflag, d_ni_ptr, d_ni_cell, d_nl_ptr, d_nl_ent, d_nl2_ptr, d_nl2_ent, d_nl2_oth, d_ne_ptr;
     DevArray<int> d_ic_cells, d_ic_ptr, d_ic_node, d_ic_slot;
     DevArray<real> d_ni_surq, d_ni_sur, d_ni_net, d_ic_capq, d_ne_sw;
     DevArray<int> d_from, d_to, d_geom;
