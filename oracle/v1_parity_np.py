@@ -517,14 +517,23 @@ def apply_recharge(state, surface, recharge, dt, dtype=np.float32):
 def advance_parity_step(state, surface, coupling, pipe, recharge, *, rainfall_rate_mps,
                         rainfall_multiplier, horton_f0_mps, horton_fc_mps, horton_k_per_s,
                         external_receiver_cell, external_q_m3ps, dt, dtype=np.float32,
-                        max_speed=5.0):
+                        max_speed=5.0, surface_mode="full_swe"):
     """v1_parity.py:430-469 -- one step in the V1 process order.
 
+    ``surface_mode="local_inertial"`` swaps the surface operator for the
+    reference's local-inertial step (engine.py:253-256, in oracle/local_inertial_np.py);
+    the sources, clamp, drainage and ledger are the same.
     Returns (state, diagnostics) with diagnostics a dict over LEDGER_FIELDS (float64).
     """
     F = _f(dtype)
     dt = F(dt)
-    state, bnet, bin_, bout = surface_flux_step(state, surface, dt, dtype)
+    if surface_mode == "local_inertial":
+        from oracle import local_inertial_np
+        state, bnet, bin_, bout = local_inertial_np.surface_step(state, surface, dt, dtype)
+    elif surface_mode == "full_swe":
+        state, bnet, bin_, bout = surface_flux_step(state, surface, dt, dtype)
+    else:
+        raise ValueError(f"unknown surface_mode {surface_mode!r}")
     state, rain, infiltration = apply_rainfall_horton(
         state, surface, rainfall_rate_mps, rainfall_multiplier,
         horton_f0_mps, horton_fc_mps, horton_k_per_s, dt, dtype)
@@ -577,8 +586,10 @@ class OracleClock(NamedTuple):
 
 
 def choose_dt(state, surface, clock: OracleClock, final_time_s, *, cfl=0.85, interval=32,
-              reuse=0.8, max_dt=1.0, dt_floor=1.0e-3, fixed_dt=None, dtype=np.float32):
+              reuse=0.8, max_dt=1.0, dt_floor=1.0e-3, fixed_dt=None, dtype=np.float32,
+              surface_mode="full_swe"):
     """parity_runner.py:124-135 -- CFL every `interval` steps, else reuse x0.8; caps; floor.
+    The local-inertial mode evaluates the gravity-wave timestep (engine.py:220-225).
 
     Returns (dt, evaluated_dt).  dt is rounded to ``dtype`` (the state precision).
     """
@@ -586,7 +597,12 @@ def choose_dt(state, surface, clock: OracleClock, final_time_s, *, cfl=0.85, int
     if fixed_dt is not None:
         return F(fixed_dt), clock.cached_dt_s
     evaluate = clock.global_step % interval == 0
-    evaluated = float(stable_timestep(state, surface, cfl, dtype)) if evaluate else clock.cached_dt_s
+    if surface_mode == "local_inertial":
+        from oracle import local_inertial_np
+        timestep = local_inertial_np.stable_timestep
+    else:
+        timestep = stable_timestep
+    evaluated = float(timestep(state, surface, cfl, dtype)) if evaluate else clock.cached_dt_s
     candidate = evaluated if evaluate else float(F(F(evaluated) * F(reuse)))
     dt = max(min(min(candidate, max_dt), final_time_s - clock.time_s), dt_floor)
     return F(dt), evaluated
@@ -594,7 +610,7 @@ def choose_dt(state, surface, clock: OracleClock, final_time_s, *, cfl=0.85, int
 
 def advance_to_time(state, clock: OracleClock, ledger, target_time_s, final_time_s,
                     surface, coupling, pipe, recharge, forcing, *, max_steps=20000,
-                    fixed_dt=None, dtype=np.float32, **dt_kw):
+                    fixed_dt=None, dtype=np.float32, surface_mode="full_swe", **dt_kw):
     """parity_runner.py:203-234 -- step while t < target (and the step budget lasts).
 
     The clock and ledger are kept in float64 (the FP64 clock/ledger the MI355X
@@ -604,7 +620,7 @@ def advance_to_time(state, clock: OracleClock, ledger, target_time_s, final_time
     steps = 0
     while clock.time_s < target_time_s and steps < max_steps:
         dt, evaluated = choose_dt(state, surface, clock, final_time_s, fixed_dt=fixed_dt,
-                                  dtype=dtype, **dt_kw)
+                                  dtype=dtype, surface_mode=surface_mode, **dt_kw)
         rate = rain_rate(forcing, clock.time_s)
         q = external_q(forcing, clock.time_s + 0.5 * float(dt))
         state, diag = advance_parity_step(
@@ -613,7 +629,7 @@ def advance_to_time(state, clock: OracleClock, ledger, target_time_s, final_time
             horton_f0_mps=forcing.horton_f0_mps, horton_fc_mps=forcing.horton_fc_mps,
             horton_k_per_s=forcing.horton_k_per_s,
             external_receiver_cell=forcing.external_receiver_cell, external_q_m3ps=q,
-            dt=dt, dtype=dtype)
+            dt=dt, dtype=dtype, surface_mode=surface_mode)
         for k, v in diag.items():
             ledger[k] = ledger.get(k, 0.0) + v
         clock = OracleClock(clock.time_s + float(dt), evaluated, clock.global_step + 1)
