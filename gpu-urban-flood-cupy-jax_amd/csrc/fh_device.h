@@ -104,6 +104,7 @@ template <typename real>
 struct KP {                 // kernel constants, pre-rounded to the state precision
     real g, eps, half_g, two_g, cfl, max_speed, tiny, capture_q, pow43, reuse, slot_coef;
     double max_dt, dt_floor, fixed_dt;
+    double g_d;              // gravity in double (products of two host scalars, g * dt)
     int interval;
     int remap;
 };

@@ -28,6 +28,10 @@ namespace fh {
 constexpr int kGhostCopy = -1;
 constexpr int kGhostReflect = -2;
 constexpr int kGhostOutflow = -3;
+// stage boundary (code 3, and any code other than 0-2 and 4): a copy ghost in
+// the full step (the V1 step has no stage values), a wall in the local-inertial
+// step, which lets water out only at codes 4 and 1
+constexpr int kGhostStage = -4;
 
 // cell -> edge table entry: (edge_position << 1) | is_right_side, -1 none,
 // -2 in slot 0 marks an inactive (or padding) cell.
@@ -123,6 +127,23 @@ __device__ __forceinline__ real cell_cfl(const KP<real>& p, const typename V4<re
     speed = speed + sqrt(p.g * hs);
     const real local = p.cfl * lc / vmax(speed, p.tiny);
     return isfinite(local) ? local : real(INFINITY);
+}
+
+// The local-inertial CFL time of one cell (equations.py:181-207): gravity-wave
+// speed only, cfl * Lc / max(sqrt(g max(h, min depth)), 1e-12), +inf where not
+// finite.
+template <typename real>
+__device__ __forceinline__ real cell_cfl_li(const KP<real>& p, const typename V4<real>::type& s, real lc) {
+    const real hs = vmax(s.x, p.eps);
+    const real c = sqrt(p.g * hs);
+    const real local = p.cfl * lc / vmax(c, p.tiny);
+    return isfinite(local) ? local : real(INFINITY);
+}
+
+template <typename real, bool LI>
+__device__ __forceinline__ real cell_time(const KP<real>& p, const typename V4<real>::type& s, real lc) {
+    if (LI) return cell_cfl_li(p, s, lc);
+    return cell_cfl(p, s, lc);
 }
 
 // Block minimum, then one atomic min per block on the bits of a non-negative
@@ -275,7 +296,7 @@ struct CellRanges {
     const int* cell1;        // [M] end of the member's real cells
 };
 
-template <typename real>
+template <typename real, bool LI>
 __global__ void __launch_bounds__(256) k_cfl(Ctl* __restrict__ ctls, KP<real> p, int parity,
         const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc, CellRanges cr) {
     __shared__ double lds[16 * 2];
@@ -286,7 +307,7 @@ __global__ void __launch_bounds__(256) k_cfl(Ctl* __restrict__ ctls, KP<real> p,
     real mn = real(INFINITY);
     const int c1 = cr.cell1[m];
     for (int c = cr.cell0[m] + (b - blk0) * blockDim.x + threadIdx.x; c < c1; c += nblk * blockDim.x) {
-        const real local = cell_cfl(p, state[c], lc[c]);
+        const real local = cell_time<real, LI>(p, state[c], lc[c]);
         if (local < mn) mn = local;
     }
     // the next step's slot (parity) holds +inf between steps and is left so
@@ -403,6 +424,154 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
     if (eval) commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
 }
 
+// ---------------------------------------------------------------------------
+// Local-inertial surface mode (triangular_backend.py:1181-1291), three launches
+// after the step's CFL pass: k_li_flux (edge discharges), k_li_donor (donor
+// limiting), k_update<..., LI> (continuity, boundary capture, momentum, then the
+// shared rain/Horton/clamp).  The discharge needs the step's dt, so the CFL
+// minimum of an evaluating step comes from k_li_cfl, launched first.
+// ---------------------------------------------------------------------------
+
+// The step's local-inertial CFL minimum over every cell of each member, on the
+// steps that evaluate it (every other step returns at once): the same slot the
+// full step's flux kernel fills.
+template <typename real>
+__global__ void __launch_bounds__(256) k_li_cfl(Ctl* __restrict__ ctls, KP<real> p, int parity,
+        const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc, CellRanges cr) {
+    __shared__ double lds[16 * 2];
+    const int b = blockIdx.x;
+    const int m = cr.seg.blk_member[b];
+    Ctl* ctl = ctls + m;
+    const Clock& ck = ctl->clk[parity];
+    if (!ck.active || !ck.eval || ctl->override_dt || p.fixed_dt > 0.0) return;
+    const int blk0 = cr.seg.mem_blk0[m], nblk = cr.seg.mem_nblk[m];
+    real mn = real(INFINITY);
+    const int c1 = cr.cell1[m];
+    for (int c = cr.cell0[m] + (b - blk0) * blockDim.x + threadIdx.x; c < c1; c += nblk * blockDim.x) {
+        const real local = cell_cfl_li(p, state[c], lc[c]);
+        if (local < mn) mn = local;
+    }
+    commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
+}
+
+// One edge's local-inertial discharge (triangular_backend.py:1196-1248 with
+// local_inertial.py:20-48): the LISFLOOD-FP update of the face discharge from
+// the water-surface slope between the two cells, friction semi-implicit;
+// outflow-only at routable boundary edges (codes 4 and 1), zero at walls
+// (codes 0, 2, 3, inactive neighbours).  Writes rec[e] = {Q, Q nx, Q ny, 0}.
+template <typename real>
+__global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
+        ForcingTables ft, const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
+        const real* __restrict__ eny, const real* __restrict__ elen,
+        const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc,
+        const real* __restrict__ man, typename V4<real>::type* __restrict__ rec) {
+    typedef typename V4<real>::type V;
+    const int b = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
+    const int m = es.seg.blk_member[b];
+    const int e = b * (int)blockDim.x + threadIdx.x;
+    const int l = el[e], r = er[e];
+    const int rs = r >= 0 ? r : l;
+    const real nx = enx[e], ny = eny[e], len = elen[e];
+    const V sl = state[l], sr = state[rs];
+    const real lcl = lc[l], lcr = lc[rs], nl = man[l], nr = man[rs];
+    const int end = es.seg.mem_end[m];
+    FH_ISSUE_LOADS_OF("s"(el), "s"(er), "s"(enx), "s"(eny), "s"(elen), "s"(state), "s"(lc), "s"(man));
+    Ctl* ctl = ctls + m;
+    const Clock ck = ctl->clk[parity];
+    const bool eval = ck.active && ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);
+    const double gmin = eval ? __longlong_as_double((long long)ctl->cfl_bits[parity]) : INFINITY;
+    const StepPlan<real> pl = plan_step<real>(ctl, ck, p, ft, m, gmin);
+    if (!pl.active || e >= end) return;
+    const real zero = real(0), half = real(0.5);
+    const bool interior = r >= 0;
+    const bool outlet = r == kGhostOutflow || r == kGhostCopy;
+    real q = zero;
+    if (interior || outlet) {
+        const real dt = (real)pl.dt;
+        const real eta_l = sl.w + sl.x;
+        const real eta_r = interior ? sr.w + sr.x : sl.w;
+        const real bed_star = interior ? vmax(sl.w, sr.w) : sl.w;
+        const real h_flow = vmax(vmax(eta_l, eta_r) - bed_star, zero);
+        const real distance = vmax(half * (lcl + (interior ? lcr : lcl)), real(1.0e-6));
+        const real slope = (eta_r - eta_l) / distance;
+        const real nm_l = sl.y * nx + sl.z * ny;
+        real nm_face = nm_l;
+        real n_face = nl;
+        if (interior) {
+            const real nm_r = sr.y * nx + sr.z * ny;
+            nm_face = half * (nm_l + nm_r);
+            n_face = half * (nl + nr);
+        }
+        const real q_old = nm_face * len;
+        const real area = vmax(h_flow * len, real(1.0e-8));
+        const real radius = vmax(h_flow, real(1.0e-6));
+        const real numerator = q_old - p.g * area * dt * slope;
+        const real gdt = (real)(p.g_d * pl.dt);
+        const real denominator = real(1) + (gdt * (n_face * n_face) * fabs(q_old) / (pow(radius, p.pow43) * area));
+        q = numerator / denominator;
+        if (!interior) q = vmax(q, zero);
+    }
+    V o;
+    o.x = q; o.y = q * nx; o.z = q * ny; o.w = zero;
+    rec[e] = o;
+}
+
+// Donor limiting of the interior transfers (network_kernels.py:45-67), one
+// thread per cell: the cell's requested outflows |Q dt| over the interior edges
+// it donates to (from-cell when Q dt >= 0, else to-cell), summed in ascending
+// edge order (perm: the cell's entry slots in that order, two bits each, 3 =
+// end), give the factor min(1, max(h, 0) A / sum); the cell then writes the
+// limited transfer sign * |Q dt| * factor of each edge it donates to (every
+// interior edge has exactly one donor) into T[edge].
+template <typename real>
+__global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
+        ForcingTables ft, const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
+        const unsigned char* __restrict__ perm, const typename V4<real>::type* __restrict__ rec,
+        real* __restrict__ T, const typename V4<real>::type* __restrict__ state, const real* __restrict__ area_) {
+    const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    const int c = b * blockDim.x + threadIdx.x;
+    const int m = seg.blk_member[b];
+    const int ent[3] = {ce0[c], ce1[c], ce2[c]};
+    real qk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) qk[k] = rec[ent[k] >= 0 ? (ent[k] >> 2) : 0].x;
+    const real h = state[c].x, A = area_[c];
+    const unsigned order = perm[c];
+    const int mend = seg.mem_end[m];
+    FH_ISSUE_LOADS_OF("s"(ce0), "s"(ce1), "s"(ce2), "s"(perm), "s"(rec), "s"(state), "s"(area_));
+    Ctl* ctl = ctls + m;
+    const Clock ck = ctl->clk[parity];
+    const bool eval = ck.active && ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);
+    const double gmin = eval ? __longlong_as_double((long long)ctl->cfl_bits[parity]) : INFINITY;
+    const StepPlan<real> pl = plan_step<real>(ctl, ck, p, ft, m, gmin);
+    if (!pl.active || c >= mend || ent[0] == kCellInactive) return;
+    const real zero = real(0), dt = (real)pl.dt;
+    real req[3];
+    bool donor[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int en = ent[k];
+        req[k] = qk[k] * dt;
+        // interior entry, and this cell is its donor: the from (left) cell when the
+        // request is >= 0, the to (right) cell otherwise
+        donor[k] = en >= 0 && !(en & 2) && ((req[k] >= zero) != ((en & 1) != 0));
+    }
+    real sum = zero;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const unsigned k = (order >> (2 * j)) & 3u;
+        if (k == 3u) break;
+        if (k == 0u ? donor[0] : (k == 1u ? donor[1] : donor[2])) {
+            const real r = k == 0u ? req[0] : (k == 1u ? req[1] : req[2]);
+            sum = sum + fabs(r);
+        }
+    }
+    const real factor = vmin(real(1), (vmax(h, zero) * A) / vmax(sum, p.tiny));
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        if (donor[k]) T[ent[k] >> 2] = (req[k] >= zero ? real(1) : real(-1)) * (fabs(req[k]) * factor);
+}
+
 // Error-free transformation a + b = s + err (TwoSum, Knuth).
 template <typename real>
 __device__ __forceinline__ real two_sum(real a, real b, real& err) {
@@ -433,14 +602,17 @@ struct CellSources {
     const real* tab_hk;
 };
 
-template <typename real, bool COMP, bool CLS>
+// LI: the local-inertial surface (triangular_backend.py:1249-1286) in place of
+// the flux divergence and friction: rec holds {Q, Q nx, Q ny} per edge and recR
+// the donor-limited transfer T of each interior edge (k_li_flux, k_li_donor).
+template <typename real, bool COMP, bool CLS, bool LI>
 __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
         ForcingTables ft,
         const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
         const typename V4<real>::type* __restrict__ rec, const real* __restrict__ recR,
         typename V4<real>::type* __restrict__ state, typename V4<real>::type* __restrict__ comp,
-        real* __restrict__ wet, const real* __restrict__ area_, CellSources<real> src,
-        double* __restrict__ part) {
+        real* __restrict__ wet, const real* __restrict__ area_, const real* __restrict__ lc_,
+        CellSources<real> src, double* __restrict__ part) {
     typedef typename V4<real>::type V;
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
     const int c = b * blockDim.x + threadIdx.x;
@@ -464,6 +636,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
         rR[k] = recR[pos];
     }
     const real A = area_[c];
+    const real lcc = LI ? lc_[c] : real(0);
     real n2, rm, f0, fc, hk;
     if (CLS) {
         const int k = src.cls[c];
@@ -509,6 +682,61 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
             wet[c] = wt;
             if (COMP) { V z; z.x = zero; z.y = zero; z.z = zero; z.w = zero; comp[c] = z; }
         } else {
+            real h, hu, hv;
+            real ch = zero, chu = zero, chv = zero;
+            if constexpr (LI) {
+                // continuity: -T at the from entries, then +T at the to entries
+                // (network_kernels.py:70-82), over the interior edges
+                real delta = zero;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int en = ent[k];
+                    if (en < 0 || (en & 2)) continue;
+                    delta = delta + ((en & 1) ? rR[k] : -rR[k]);
+                }
+                const real vol_after = vmax(vmax(s.x, zero) * A + delta, zero);
+                // boundary outflow max(Q, 0) dt at this cell's boundary edges (zero at
+                // walls), limited by the volume left (triangular_backend.py:1767-1772)
+                real req[3];
+                real bsum = zero;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int en = ent[k];
+                    req[k] = (en >= 0 && (en & 2)) ? vmax(rr[k].x, zero) * dt : zero;
+                    if (en >= 0 && (en & 2)) bsum = bsum + req[k];
+                }
+                const real bfac = vmin(real(1), vol_after / vmax(bsum, p.tiny));
+                real out = zero;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int en = ent[k];
+                    if (en < 0 || !(en & 2)) continue;
+                    const real o = req[k] * bfac;
+                    out = out + o;
+                    bout += (double)o;
+                }
+                const real vdelta = delta - out;
+                if (COMP) {
+                    h = two_sum<real>(s.x, vdelta / A + cc.x, ch);
+                    if (!(h > zero)) { h = vmax(h, zero); ch = zero; }
+                } else {
+                    h = vmax(s.x + vdelta / A, zero);
+                }
+                // momentum from the edge discharges around the cell
+                // (triangular_backend.py:1270-1285): from entries, then to entries
+                real Lu = zero, Lv = zero, Ru = zero, Rv = zero;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int en = ent[k];
+                    if (en < 0) continue;
+                    if (en & 1) { Ru = Ru + rr[k].y; Rv = Rv + rr[k].z; }
+                    else { Lu = Lu + rr[k].y; Lv = Lv + rr[k].z; }
+                }
+                const real perimeter = real(2) * A / vmax(lcc, p.tiny);
+                const bool w = h > eps;
+                hu = w ? (Lu + Ru) / vmax(perimeter, p.tiny) : zero;
+                hv = w ? (Lv + Rv) / vmax(perimeter, p.tiny) : zero;
+            } else {
             real Lh = zero, Lu = zero, Lv = zero, Rh = zero, Ru = zero, Rv = zero;
             bool anyR = false;
 #pragma unroll
@@ -530,8 +758,6 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
             const real dh = anyR ? Lh + Rh : Lh;
             const real dhu = anyR ? Lu + Ru : Lu;
             const real dhv = anyR ? Lv + Rv : Lv;
-            real h, hu, hv;
-            real ch = zero, chu = zero, chv = zero;
             if (COMP) {
                 h = two_sum<real>(s.x, dt * dh / A + cc.x, ch);
                 hu = two_sum<real>(s.y, dt * dhu / A + cc.y, chu);
@@ -551,6 +777,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
                 const real damping = real(1) + dt * p.g * n2 * speed / pow(hs, p.pow43);
                 if (w) { hu = hu / damping; hv = hv / damping; if (COMP) { chu = chu / damping; chv = chv / damping; } }
                 else { hu = zero; hv = zero; chu = zero; chv = zero; }
+            }
             }
             // rain + Horton infiltration (capacity from the wet time of the previous step)
             const real rain_depth = vmax(rate, zero) * rm * dt;

@@ -169,7 +169,7 @@ struct Member {
     std::vector<int> el, er;
     std::vector<real> nx, ny, len;
     std::vector<int> ce0, ce1, ce2;     // local edge positions, encoded
-    std::vector<real> area, n2, lc, bed;
+    std::vector<real> area, n2, n1, lc, bed;
     std::vector<uint8_t> active;
     std::vector<real> rainm, f0, fc, hk;
     std::vector<float> rain_s, rain_e, rain_r;
@@ -382,6 +382,10 @@ public:
         kp.dt_floor = P.dt_floor;
         kp.fixed_dt = P.fixed_dt > 0.0 ? (double)(real)P.fixed_dt : 0.0;
         kp.interval = std::max(1, P.cfl_interval);
+        kp.g_d = P.gravity;
+        if (P.surface_mode != 0 && P.surface_mode != 1) throw ValueError{"surface_mode must be 0 or 1"};
+        li = P.surface_mode == 1;
+        if (li) ept = 1;
         kp.remap = P.xcd_remap;
         FH_CHECK(hipStreamSynchronize(stream));
     }
@@ -432,7 +436,8 @@ public:
             int kind;
             if (cd == 0 || cd == 2) kind = fh::kGhostReflect;
             else if (!in && cd == 4) kind = fh::kGhostOutflow;
-            else kind = fh::kGhostCopy;
+            else if (cd == 1) kind = fh::kGhostCopy;
+            else kind = fh::kGhostStage;
             ghost.push_back(e);
             ghost_kind.push_back(kind);
         }
@@ -477,12 +482,13 @@ public:
             if (!act(c)) t[0] = fh::kCellInactive;
             s.ce0[c] = t[0]; s.ce1[c] = t[1]; s.ce2[c] = t[2];
         }
-        s.area.assign(C_, 0); s.n2.assign(C_, 0); s.lc.assign(C_, 0); s.bed.assign(C_, 0);
+        s.area.assign(C_, 0); s.n2.assign(C_, 0); s.n1.assign(C_, 0); s.lc.assign(C_, 0); s.bed.assign(C_, 0);
         s.active.assign(C_, 1);
         for (int64_t c = 0; c < C_; ++c) {
             const real nn = (real)n[c];
             s.area[c] = (real)area[c];
             s.n2[c] = nn * nn;
+            s.n1[c] = nn;
             s.lc[c] = (real)lc_[c];
             s.bed[c] = (real)bed[c];
             s.active[c] = act(c) ? 1 : 0;
@@ -886,6 +892,37 @@ public:
         d_el.set(el, s); d_er.set(er, s); d_nx.set(nx, s); d_ny.set(ny, s); d_len.set(len, s);
         d_rec.alloc(std::max(edges.total, 1)); d_recR.alloc(std::max(edges.total, 1));
         d_ce0.set(ce0, s); d_ce1.set(ce1, s); d_ce2.set(ce2, s);
+        if (li) {
+            // Manning n, and each cell's interior entry slots in ascending edge order
+            // (two bits per slot, 3 = end) for the donor sums
+            std::vector<real> man(cells.total, 0);
+            std::vector<unsigned char> perm(cells.total, 0xFF);
+            for (int i = 0; i < M; ++i) {
+                const Member<real>& m = mem[order[i]];
+                const int cb = cells.base[i];
+                for (int64_t c = 0; c < m.C; ++c) {
+                    const int64_t q = cb + m.cp((int)c);
+                    man[q] = m.n1[c];
+                    const int t[3] = {m.ce0[c], m.ce1[c], m.ce2[c]};
+                    int slot[3], key[3], n = 0;
+                    for (int k = 0; k < 3; ++k)
+                        if (t[k] >= 0 && (t[k] >> 1) < m.E_int) { slot[n] = k; key[n] = t[k] >> 1; ++n; }
+                    for (int a = 1; a < n; ++a)
+                        for (int b2 = a; b2 > 0 && key[b2 - 1] > key[b2]; --b2) {
+                            std::swap(key[b2 - 1], key[b2]);
+                            std::swap(slot[b2 - 1], slot[b2]);
+                        }
+                    unsigned code = 0xFF;
+                    for (int k = 0; k < n; ++k) code = (code & ~(3u << (2 * k))) | ((unsigned)slot[k] << (2 * k));
+                    perm[q] = (unsigned char)code;
+                }
+            }
+            d_cell_n.set(man, s);
+            d_perm.set(perm, s);
+        } else {
+            d_cell_n.release();
+            d_perm.release();
+        }
         d_state.set(st, s); d_wet.set(wet, s); d_area.set(area, s); d_n2.set(n2, s); d_lc.set(lc, s);
         d_rainm.set(rainm, s); d_f0.set(f0, s); d_fc.set(fc, s); d_k.set(hk, s);
         build_source_classes(n2, rainm, f0, fc, hk);
@@ -1433,8 +1470,27 @@ public:
 
     // CFL minimum over every cell of each member into cfl_min (stable_timestep)
     void launch_cfl() {
-        go(fh::k_cfl<real>, begin_l.blocks, 256, ctl, kp, par, d_state.p, d_lc.p,
+        if (li)
+            go(fh::k_cfl<real, true>, begin_l.blocks, 256, ctl, kp, par, d_state.p, d_lc.p,
+               fh::CellRanges{seg_begin.seg(), d_cell0.p, d_cell1.p});
+        else
+            go(fh::k_cfl<real, false>, begin_l.blocks, 256, ctl, kp, par, d_state.p, d_lc.p,
+               fh::CellRanges{seg_begin.seg(), d_cell0.p, d_cell1.p});
+    }
+    // local-inertial surface: the step's CFL pass, edge discharges, donor limiting
+    void launch_li_cfl() {
+        go(fh::k_li_cfl<real>, begin_l.blocks, 256, ctl, kp, par, d_state.p, d_lc.p,
            fh::CellRanges{seg_begin.seg(), d_cell0.p, d_cell1.p});
+    }
+    void launch_li_flux() {
+        if (edges.blocks == 0) return;
+        fh::EdgeSeg es{seg_edges.seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
+        go(fh::k_li_flux<real>, edges.blocks, bs, ctl, kp, es, par, forcing_tables(), d_el.p, d_er.p, d_nx.p,
+           d_ny.p, d_len.p, d_state.p, d_lc.p, d_cell_n.p, d_rec.p);
+    }
+    void launch_li_donor() {
+        go(fh::k_li_donor<real>, cells.blocks, bs, ctl, kp, seg_cells.seg(), par, forcing_tables(), d_ce0.p,
+           d_ce1.p, d_ce2.p, d_perm.p, d_rec.p, d_recR.p, d_state.p, d_area.p);
     }
     void launch_flux() {
         if (edges.blocks + orphans.blocks == 0) return;
@@ -1450,15 +1506,18 @@ public:
     }
     void launch_update() {
         const fh::CellSources<real> src{d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p, d_cls.p, d_cls_tab.p, d_cls_hk.p};
-#define FH_UPDATE(COMP_, CLS_)                                                                              \
-        go(fh::k_update<real, COMP_, CLS_>, cells.blocks, bs, ctl, kp, seg_cells.seg(), par, forcing_tables(),    \
-           d_ce0.p, d_ce1.p, d_ce2.p,                                                                           \
-           d_rec.p, d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, src, d_part_upd.p)
+#define FH_UPDATE(COMP_, CLS_, LI_)                                                                         \
+        go(fh::k_update<real, COMP_, CLS_, LI_>, cells.blocks, bs, ctl, kp, seg_cells.seg(), par,                 \
+           forcing_tables(), d_ce0.p, d_ce1.p, d_ce2.p,                                                         \
+           d_rec.p, d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, d_lc.p, src, d_part_upd.p)
+#define FH_UPDATE_M(COMP_, CLS_) \
+        if (li) { FH_UPDATE(COMP_, CLS_, true); } else { FH_UPDATE(COMP_, CLS_, false); }
         if (P.compensated) {
-            if (use_classes) FH_UPDATE(true, true); else FH_UPDATE(true, false);
+            if (use_classes) { FH_UPDATE_M(true, true); } else { FH_UPDATE_M(true, false); }
         } else {
-            if (use_classes) FH_UPDATE(false, true); else FH_UPDATE(false, false);
+            if (use_classes) { FH_UPDATE_M(false, true); } else { FH_UPDATE_M(false, false); }
         }
+#undef FH_UPDATE_M
 #undef FH_UPDATE
     }
 
@@ -1493,7 +1552,13 @@ public:
     // One step: flux and update on the clock of parity `par`; the update writes
     // the next step's clock into the other slot, so the parity flips per step.
     void launch_step() {
-        launch_flux();
+        if (li) {
+            launch_li_cfl();
+            launch_li_flux();
+            launch_li_donor();
+        } else {
+            launch_flux();
+        }
         launch_update();
         launch_drainage([](const char*, double, auto&& fn) { fn(); });
         par ^= 1;
@@ -1689,7 +1754,13 @@ public:
                                  (P.compensated ? Ct * 8 * R4 : 0.0);
         fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, par, INFINITY, INFINITY, (int64_t)steps + 8);
         for (int i = 0; i < steps; ++i) {
-            timed("flux", flux_bytes, [&] { launch_flux(); });
+            if (li) {
+                timed("li_cfl", 0.0, [&] { launch_li_cfl(); });
+                timed("li_flux", flux_bytes, [&] { launch_li_flux(); });
+                timed("li_donor", Ct * (3 * I4 + 1 + 2 * R4) + Et * 2 * R4, [&] { launch_li_donor(); });
+            } else {
+                timed("flux", flux_bytes, [&] { launch_flux(); });
+            }
             timed("update", upd_bytes, [&] { launch_update(); });
             launch_drainage(timed);
             par ^= 1;
@@ -1732,6 +1803,9 @@ public:
     DevArray<real> d_nr_area;
     DevArray<int> d_cell0, d_cell1, d_rain_off, d_rain_n, d_ext_off, d_ext_n, d_ocells;
     DevArray<int> d_el, d_er, d_ce0, d_ce1, d_ce2;
+    DevArray<real> d_cell_n;          // local-inertial: Manning n per cell
+    DevArray<unsigned char> d_perm;   // local-inertial: entry slots in ascending edge order
+    bool li = false;                  // local-inertial surface mode
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
     DevArray<unsigned short> d_cls;   // source class per cell (when use_classes)
     DevArray<V> d_cls_tab;            // class -> {n^2, m, f0, fc}
@@ -1830,6 +1904,7 @@ void fh_default_params(fh_params* p) {
     p->pack_order = 0;
     p->cell_order = 1;
     p->source_classes = 1;
+    p->surface_mode = 0;
 }
 
 const char* fh_version(void) { return "floodhip 0.2.0 (gfx950)"; }

@@ -81,12 +81,17 @@ class SolverOptions(NamedTuple):
     pack_order: int = 0
     cell_order: int = 1
     source_classes: int = 1
+    surface_mode: str = "full_swe"   # or "local_inertial" (engine.py:219-256)
     cfl: float = 0.85
     cfl_interval: int = 32
     reuse_factor: float = 0.8
     max_dt: float = 1.0
     dt_floor: float = 1.0e-3
     max_speed: float = 5.0
+
+
+# config.py:157-158 of the reference: the surface operators of the step
+SURFACE_MODES = {"full_swe": 0, "local_inertial": 1}
 
 
 class HipParitySolver:
@@ -121,6 +126,9 @@ class HipParitySolver:
         p.pack_order = int(opts.pack_order)
         p.cell_order = int(opts.cell_order)
         p.source_classes = int(opts.source_classes)
+        if opts.surface_mode not in SURFACE_MODES:
+            raise ValueError(f"surface_mode must be one of {sorted(SURFACE_MODES)}, got {opts.surface_mode!r}")
+        p.surface_mode = SURFACE_MODES[opts.surface_mode]
         self._ctx = self.lib.fh_create_batch(C.byref(p), int(members))
         if not self._ctx:
             msg = self.lib.fh_global_error().decode()

@@ -72,6 +72,9 @@ typedef struct {
     int32_t source_classes;     /* 1: store the per-cell source tuples (Manning n^2, rain multiplier, Horton
                                    f0, fc, k) as a 16-bit class per cell plus a class table when there are
                                    at most 65,535 distinct tuples; 0: one array each.  Same values. */
+    int32_t surface_mode;       /* 0: full shallow-water step (v1_parity.py:161-214); 1: local-inertial step
+                                   (triangular_backend.py:1181-1291) with the gravity-wave CFL timestep
+                                   (engine.py:220-225).  Sources, clamp, drainage and ledger are shared. */
 } fh_params;
 
 typedef struct {
