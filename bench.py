@@ -129,6 +129,36 @@ def batch_leg(base, args, plain_rate):
             "options": opts, "autotune": tuned, "kernels": kern}
 
 
+def local_inertial_leg(inp, args):
+    """The reference's production surface mode (surface_step_mode: local_inertial,
+    engine.py:219-256) on the same workload and network: the step's CFL pass, the
+    edge discharges, the donor limiting and the update, then the same drainage."""
+    from floodhip.solver import HipParitySolver
+    C = int(inp.surface.cell_area.shape[0])
+    s = HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
+                        surface_mode="local_inertial", use_graph=not args.no_graph)
+    final_t = 3600.0
+    s.advance_to(final_t, final_t, max_steps=max(args.warmup, 1), raise_on_budget=False)
+    c0 = s.clock()
+    t0 = time.perf_counter()
+    c1, _ = s.advance_to(final_t, final_t, max_steps=args.li_steps, raise_on_budget=False)
+    el = time.perf_counter() - t0
+    steps = c1["global_step"] - c0["global_step"]
+    sim = c1["time_s"] - c0["time_s"]
+    st = s.get_state()
+    ok = bool(np.isfinite(st.h).all() and st.h.min() >= -1e-7)
+    prof = s.profile_kernels(steps=args.profile_steps)
+    s.close()
+    log(f"local-inertial: {steps} steps in {el:.3f}s: {1e3 * el / steps:.4f} ms/step, "
+        f"{C * steps / el:.4e} tri-steps/s, {sim / el:.1f} simulated s per s (mean dt {sim / steps:.3f}s)")
+    for k in prof:
+        log(f"kernel {k['kernel']:>18s}: {1e3 * k['avg_ms']:8.2f} us  x{k['launches']}")
+    return {"workload": "the same 981,880-triangle coupled workload, surface_mode local_inertial, float32, hipGraph",
+            "value": C * steps / el, "unit": "triangle-steps/s", "steps": steps, "ms_per_step": 1e3 * el / steps,
+            "simulated_s_per_wall_s": sim / el, "mean_dt_s": sim / steps, "state_check": ok,
+            "kernels": {k["kernel"]: {"avg_us": round(1e3 * k["avg_ms"], 3)} for k in prof}}
+
+
 def batch_uneven_leg(args):
     from floodhip import synth
     from floodhip.autotune import autotune
@@ -166,6 +196,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-uneven", action="store_true", help="skip the 48-member uneven ensemble")
+    ap.add_argument("--no-li", action="store_true", help="skip the local-inertial surface mode leg")
+    ap.add_argument("--li-steps", type=int, default=1000)
     ap.add_argument("--copy-probe-mib", type=int, default=256,
                     help="size of the streaming-copy probe (achievable HBM bandwidth); 0 skips it")
     args = ap.parse_args()
@@ -219,6 +251,7 @@ def main():
                                       if survey_bytes(k["kernel"], E, C, args.compensated) else None)}
                for k in prof}
     s.close()
+    local_inertial = None if args.no_li else local_inertial_leg(inp, args)
     batched = None
     if not args.no_batch and args.batch_members > 0:
         batched = batch_leg(inp, args, tri_steps)
@@ -264,6 +297,8 @@ def main():
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None if args.compensated else measured_traffic(dom["kernel"])},
         "kernels": kernels,
+        "simulated_s_per_wall_s": (c1["time_s"] - c0["time_s"]) / el,
+        "local_inertial": local_inertial,
         "copy_probe": copy,
         "batched_shared_mesh": batched,
         "batched_uneven": uneven,
