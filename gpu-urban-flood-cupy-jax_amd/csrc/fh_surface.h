@@ -458,13 +458,14 @@ __global__ void __launch_bounds__(256) k_li_cfl(Ctl* __restrict__ ctls, KP<real>
 // local_inertial.py:20-48): the LISFLOOD-FP update of the face discharge from
 // the water-surface slope between the two cells, friction semi-implicit;
 // outflow-only at routable boundary edges (codes 4 and 1), zero at walls
-// (codes 0, 2, 3, inactive neighbours).  Writes rec[e] = {Q, Q nx, Q ny, 0}.
+// (codes 0, 2, 3, inactive neighbours).  Writes rec[e] = {Q, Q nx, Q ny, 0} for
+// the update and qv[e] = Q, which the donor pass gathers (4 bytes, not 16).
 template <typename real>
 __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
         ForcingTables ft, const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
         const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc,
-        const real* __restrict__ man, typename V4<real>::type* __restrict__ rec) {
+        const real* __restrict__ man, typename V4<real>::type* __restrict__ rec, real* __restrict__ qv) {
     typedef typename V4<real>::type V;
     const int b = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
     const int m = es.seg.blk_member[b];
@@ -514,6 +515,7 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
     V o;
     o.x = q; o.y = q * nx; o.z = q * ny; o.w = zero;
     rec[e] = o;
+    qv[e] = q;
 }
 
 // Donor limiting of the interior transfers (network_kernels.py:45-67), one
@@ -526,7 +528,7 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
 template <typename real>
 __global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
         ForcingTables ft, const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
-        const unsigned char* __restrict__ perm, const typename V4<real>::type* __restrict__ rec,
+        const unsigned char* __restrict__ perm, const real* __restrict__ qv,
         real* __restrict__ T, const typename V4<real>::type* __restrict__ state, const real* __restrict__ area_) {
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
     const int c = b * blockDim.x + threadIdx.x;
@@ -534,11 +536,11 @@ __global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<rea
     const int ent[3] = {ce0[c], ce1[c], ce2[c]};
     real qk[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) qk[k] = rec[ent[k] >= 0 ? (ent[k] >> 2) : 0].x;
+    for (int k = 0; k < 3; ++k) qk[k] = qv[ent[k] >= 0 ? (ent[k] >> 2) : 0];
     const real h = state[c].x, A = area_[c];
     const unsigned order = perm[c];
     const int mend = seg.mem_end[m];
-    FH_ISSUE_LOADS_OF("s"(ce0), "s"(ce1), "s"(ce2), "s"(perm), "s"(rec), "s"(state), "s"(area_));
+    FH_ISSUE_LOADS_OF("s"(ce0), "s"(ce1), "s"(ce2), "s"(perm), "s"(qv), "s"(state), "s"(area_));
     Ctl* ctl = ctls + m;
     const Clock ck = ctl->clk[parity];
     const bool eval = ck.active && ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);

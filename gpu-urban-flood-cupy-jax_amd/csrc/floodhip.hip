@@ -919,9 +919,11 @@ public:
             }
             d_cell_n.set(man, s);
             d_perm.set(perm, s);
+            d_qli.alloc(std::max(edges.total, 1));
         } else {
             d_cell_n.release();
             d_perm.release();
+            d_qli.release();
         }
         d_state.set(st, s); d_wet.set(wet, s); d_area.set(area, s); d_n2.set(n2, s); d_lc.set(lc, s);
         d_rainm.set(rainm, s); d_f0.set(f0, s); d_fc.set(fc, s); d_k.set(hk, s);
@@ -1486,11 +1488,11 @@ public:
         if (edges.blocks == 0) return;
         fh::EdgeSeg es{seg_edges.seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
         go(fh::k_li_flux<real>, edges.blocks, bs, ctl, kp, es, par, forcing_tables(), d_el.p, d_er.p, d_nx.p,
-           d_ny.p, d_len.p, d_state.p, d_lc.p, d_cell_n.p, d_rec.p);
+           d_ny.p, d_len.p, d_state.p, d_lc.p, d_cell_n.p, d_rec.p, d_qli.p);
     }
     void launch_li_donor() {
         go(fh::k_li_donor<real>, cells.blocks, bs, ctl, kp, seg_cells.seg(), par, forcing_tables(), d_ce0.p,
-           d_ce1.p, d_ce2.p, d_perm.p, d_rec.p, d_recR.p, d_state.p, d_area.p);
+           d_ce1.p, d_ce2.p, d_perm.p, d_qli.p, d_recR.p, d_state.p, d_area.p);
     }
     void launch_flux() {
         if (edges.blocks + orphans.blocks == 0) return;
@@ -1805,6 +1807,7 @@ public:
     DevArray<int> d_el, d_er, d_ce0, d_ce1, d_ce2;
     DevArray<real> d_cell_n;          // local-inertial: Manning n per cell
     DevArray<unsigned char> d_perm;   // local-inertial: entry slots in ascending edge order
+    DevArray<real> d_qli;             // local-inertial: edge discharge, for the donor pass
     bool li = false;                  // local-inertial surface mode
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
     DevArray<unsigned short> d_cls;   // source class per cell (when use_classes)
