@@ -15,3 +15,11 @@ class BackendUnavailableError(OperationalError):
 
 class SimulationError(OperationalError):
     """The numerical run failed validation."""
+
+
+class RainfallInputError(OperationalError):
+    """Rainfall input is invalid or incomplete."""
+
+
+class DataFileMissingError(OperationalError):
+    """A required immutable model asset is unavailable."""

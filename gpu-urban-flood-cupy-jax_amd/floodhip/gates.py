@@ -14,6 +14,8 @@ from pathlib import Path
 
 import numpy as np
 
+from .errors import ConfigurationError
+
 DEPTH_THRESHOLD_M = 0.05
 LEDGER_TERMS = {            # candidate report field -> reference report field (parity_validation.py:15-23)
     "rainfall_input_m3": "rainfall_input_m3",
@@ -115,11 +117,39 @@ def _load_dir(directory: Path) -> dict:
     return out
 
 
-def validate_run_dirs(reference_dir, candidate_dir, area, threshold_m=DEPTH_THRESHOLD_M) -> dict:
-    """Gates over two run directories in the reference's layout (parity_validation.py:43-93)."""
-    ref = _load_dir(Path(reference_dir))
-    cand = _load_dir(Path(candidate_dir))
-    valid = cand.get("active_surface_mask")
-    if valid is not None and "building_mask" in cand:
-        valid = valid & ~cand["building_mask"]
+def validate_run_dirs(reference_dir, candidate_dir, area=None, threshold_m=DEPTH_THRESHOLD_M) -> dict:
+    """Gates over two run directories in the reference's layout (parity_validation.py:43-93).
+
+    Checks what the reference's validator checks before the gates: both archives,
+    snapshot times and depths of the same shapes, the candidate's active mask
+    (minus its building mask) and centroids.  Without ``area`` the cell areas
+    come from the topology asset named in the candidate report's input report,
+    as the reference recovers them."""
+    ref_dir, cand_dir = Path(reference_dir), Path(candidate_dir)
+    for d in (ref_dir, cand_dir):
+        if not (d / "flood_simulation_outputs.npz").is_file():
+            raise ConfigurationError("Reference and candidate must both contain flood_simulation_outputs.npz.")
+    ref = _load_dir(ref_dir)
+    cand = _load_dir(cand_dir)
+    for run in (ref, cand):
+        if not {"snapshot_times_s", "depth_snapshots_m"}.issubset(run):
+            raise ConfigurationError("Parity archives require snapshot_times_s and depth_snapshots_m.")
+    if ref["depth_snapshots_m"].shape != cand["depth_snapshots_m"].shape or \
+            ref["snapshot_times_s"].shape != cand["snapshot_times_s"].shape:
+        raise ConfigurationError("Parity shape mismatch between reference and candidate.")
+    if "active_surface_mask" not in cand:
+        raise ConfigurationError("Candidate archive lacks active_surface_mask.")
+    valid = cand["active_surface_mask"].astype(bool)
+    if "building_mask" in cand:
+        valid = valid & ~cand["building_mask"].astype(bool)
+    if valid.shape != ref["depth_snapshots_m"].shape[1:]:
+        raise ConfigurationError("Candidate domain mask does not match triangle snapshots.")
+    if "centroids_world" not in cand:
+        raise ConfigurationError("Candidate archive lacks exact-topology centroids_world.")
+    if area is None:
+        topology = Path(cand["report"]["input_report"]["assets"]["assets"]["topology"]["path"])
+        with np.load(topology, allow_pickle=False) as t:
+            area = t["surface_cell_area_m2"].astype(np.float64)
+    if np.asarray(area).shape != valid.shape:
+        raise ConfigurationError("Production triangle areas do not match parity output.")
     return validate(ref, cand, area, valid, threshold_m)

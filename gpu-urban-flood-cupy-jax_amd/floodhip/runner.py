@@ -86,14 +86,34 @@ def run_hip_v1_parity(inputs: LoadedParityInputs, *, duration_s: float, max_step
               + led["unreturned_overflow"] + led["recharge_loss"] - led["boundary_net_flux"])
     rhs_complete = rhs_v1 + final_recharge
     clock = s.clock()
+    steps = int(clock["global_step"])
+    valid = np.asarray(inputs.surface.active, bool)
+    if "building_mask" in inputs.host:
+        valid = valid & ~np.asarray(inputs.host["building_mask"], bool)
+    mode = s.options.surface_mode
+    budget = int(len(requested) * max_steps_per_hour)
+    # the reference's report fields (parity_runner.py:346-388), with this backend's values
     report = {
-        "implementation": "floodhip_mi355x", "backend": "hip_gfx950", "is_gpu": True,
-        "surface_step_mode": "full_swe", "drainage_mode": "v1_full_dynamic_network",
-        "duration_s": float(duration_s), "runtime_s": runtime, "chunk_runtimes_s": chunk_rt,
-        "step_count": int(clock["global_step"]), "steps_at_snapshot": steps_at,
+        "implementation": "floodhip_mi355x", "model_version": "0.3.0-parity",
+        "model_variant": "hip_v1_parity" if mode == "full_swe" else "hip_local_inertial",
+        "topology_mode": "exact_production", "surface_step_mode": mode,
+        "drainage_mode": "v1_full_dynamic_network",
+        "boundary_condition": "v1_code4_outflow_only_with_historical_external_inflow",
+        "backend": "hip_gfx950", "is_gpu": True, "devices": [f"hip:{int(s.options.device)}"],
+        "duration_min": float(duration_s) / 60.0, "duration_s": float(duration_s), "runtime_s": runtime,
+        "loop_mode": "while", "maximum_steps_per_output_interval": int(max_steps_per_hour),
+        "compiled_loop_iteration_budget": budget, "executed_loop_iterations": steps,
+        "inactive_tail_iterations_avoided": budget - steps,
+        "chunk_runtimes_s": chunk_rt, "first_chunk_compile_and_execution_s": chunk_rt[0],
+        "remaining_chunks_execution_s": float(sum(chunk_rt[1:])),
+        "step_count": steps, "steps_at_snapshot": steps_at,
+        "steps_per_snapshot_interval": np.diff(np.asarray([0, *steps_at], np.int64)).tolist(),
         "cfl_eval_count": int(clock["cfl_evaluations"]),
         "snapshot_count": len(times), "max_depth_m": float(np.max(depth_a)),
+        "wet_triangle_count_005m": int(np.sum((depth_a[-1] >= 0.05) & valid)),
         "drainage_node_count": int(node_a.shape[1]),
+        "drainage_link_count": int(np.asarray(inputs.pipe.link_from).size),
+        "mapped_inlet_count": int(np.asarray(inputs.coupling.inlet_cell).size),
         "initial_surface_volume_m3": initial_surface, "initial_node_volume_m3": initial_node,
         "final_surface_volume_m3": final_surface, "final_node_volume_m3": final_node,
         "final_recharge_storage_m3": final_recharge,
@@ -102,7 +122,8 @@ def run_hip_v1_parity(inputs: LoadedParityInputs, *, duration_s: float, max_step
         "v1_compatible_mass_residual_ratio": (lhs - rhs_v1) / max(abs(lhs), 1.0),
         "complete_mass_residual_m3": lhs - rhs_complete,
         "complete_mass_residual_ratio": (lhs - rhs_complete) / max(abs(lhs), 1.0),
-        "validation_status": "parity_candidate_unvalidated",
+        "validation_status": "parity_candidate_unvalidated", "reference_model": "v1_cuda_reference",
+        "observationally_validated": False, "input_report": inputs.report,
     }
     out = {"snapshot_times_s": np.asarray(times, np.float32), "steps_at_snapshot": np.asarray(steps_at, np.int64),
            "depth_snapshots_m": depth_a, "speed_snapshots_mps": np.stack(speed),
@@ -115,12 +136,24 @@ def run_hip_v1_parity(inputs: LoadedParityInputs, *, duration_s: float, max_step
 
 
 def publish(result: dict, destination, inputs: LoadedParityInputs) -> Path:
-    """Write the reference's run-directory layout atomically (parity_runner.py:401-436)."""
+    """Write the reference's run-directory layout atomically (parity_runner.py:401-436):
+    ``flood_simulation_outputs.npz`` with the reference's array names, and
+    ``run_report.json``.  Host arrays the reference writes (mesh, masks, drain node
+    positions) are included when the inputs carry them."""
     dest = Path(destination).resolve()
     if dest.exists():
         raise ConfigurationError(f"Parity output directory already exists: {dest}")
     partial = dest.parent / f".{dest.name}.partial-{uuid.uuid4().hex}"
     partial.mkdir(parents=True, exist_ok=False)
+    host = inputs.host
+    n_cells = np.asarray(inputs.surface.cell_area).shape[0]
+
+    def first(*names):
+        for n in names:
+            if n in host:
+                return np.asarray(host[n])
+        return None
+
     try:
         depth = result["depth_snapshots_m"]
         st = result.get("final_state")
@@ -133,19 +166,39 @@ def publish(result: dict, destination, inputs: LoadedParityInputs) -> Path:
             max_speed_mps=np.max(result["speed_snapshots_mps"], axis=0),
             final_node_volume_m3=result["drain_node_volume_snapshots_m3"][-1],
             active_surface_mask=np.asarray(inputs.surface.active, bool),
-            terrain_elevation_m=np.asarray(inputs.surface.bed, np.float32),
+            building_mask=np.asarray(first("building_mask") if "building_mask" in host
+                                     else np.zeros(n_cells, bool), bool),
+            road_mask=np.asarray(first("road_mask") if "road_mask" in host else np.zeros(n_cells, bool), bool),
+            terrain_elevation_m=np.asarray(first("terrain_elevation_m") if "terrain_elevation_m" in host
+                                           else inputs.surface.bed, np.float32),
+            drain_link_from=np.asarray(inputs.pipe.link_from, np.int32),
+            drain_link_to=np.asarray(inputs.pipe.link_to, np.int32),
         )
         if st is not None:
-            arrays.update(final_hu_m2ps=st.hu, final_hv_m2ps=st.hv, final_link_flow_m3ps=st.link_flow)
-        if "triangles" in inputs.host:
-            arrays["triangles"] = np.asarray(inputs.host["triangles"], np.int32)
-        if "centroids" in inputs.host:
-            arrays["centroids_world"] = np.asarray(inputs.host["centroids"], np.float32)
+            arrays.update(final_hu_m2ps=np.asarray(st.hu, np.float32), final_hv_m2ps=np.asarray(st.hv, np.float32),
+                          final_link_flow_m3ps=np.asarray(st.link_flow, np.float32))
+        for key, names, dtype in (("nodes_world", ("nodes_world", "nodes"), np.float32),
+                                  ("triangles", ("triangles",), np.int32),
+                                  ("centroids_world", ("centroids_world", "centroids"), np.float32),
+                                  ("drain_node_x_m", ("node_x_m",), np.float32),
+                                  ("drain_node_y_m", ("node_y_m",), np.float32)):
+            v = first(*names)
+            if v is not None:
+                arrays[key] = v.astype(dtype)
         np.savez_compressed(partial / "flood_simulation_outputs.npz", **arrays)
-        (partial / "run_report.json").write_text(json.dumps(result["report"], indent=2), encoding="utf-8")
+        (partial / "run_report.json").write_text(json.dumps(result["report"], indent=2, default=_json_default),
+                                                 encoding="utf-8")
         partial.rename(dest)
     except Exception:
         if partial.exists():
             shutil.rmtree(partial)
         raise
     return dest
+
+
+def _json_default(value):
+    if isinstance(value, (np.integer, np.floating, np.bool_)):
+        return value.item()
+    if isinstance(value, np.ndarray):
+        return value.tolist()
+    raise TypeError(f"not JSON serializable: {type(value).__name__}")

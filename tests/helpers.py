@@ -133,7 +133,8 @@ def stoker_depth(x, t, h_l, h_r, x0, g=9.80665):
     return h
 
 
-def oracle_run(inputs, duration_s, snapshot_interval_s=3600.0, max_steps_per_hour=20_000):
+def oracle_run(inputs, duration_s, snapshot_interval_s=3600.0, max_steps_per_hour=20_000,
+               surface_mode="full_swe"):
     """The oracle's whole-event run in the layout of ``run_hip_v1_parity``: snapshots
     at the same output targets (the first at 1e-6 s), FP64 ledger, the report fields
     the gates read (parity_runner.py:264-361 driver, v1-compatible residual)."""
@@ -142,14 +143,19 @@ def oracle_run(inputs, duration_s, snapshot_interval_s=3600.0, max_steps_per_hou
     area = np.asarray(inputs.surface.cell_area, np.float64)
     clock = orc.OracleClock(0.0, 1.0, 0)
     ledger = orc.zero_ledger()
-    depth, times, nodes = [], [], []
+    depth, times, nodes, speed, steps = [], [], [], [], []
     for i, req in enumerate(output_targets(duration_s, snapshot_interval_s)):
         target = np.float32(1.0e-6) if i == 0 else req
         st, clock, ledger = orc.advance_to_time(st, clock, ledger, float(target), float(duration_s), s, c, p, r, f,
-                                                max_steps=max_steps_per_hour)
+                                                max_steps=max_steps_per_hour, surface_mode=surface_mode)
         depth.append(np.asarray(st.h, np.float32))
         nodes.append(np.asarray(st.node_volume, np.float32))
         times.append(clock.time_s)
+        steps.append(clock.global_step)
+        wet = st.h > 1.0e-4
+        sp = np.zeros(st.h.shape, np.float32)
+        sp[wet] = np.sqrt(st.hu[wet] ** 2 + st.hv[wet] ** 2) / st.h[wet]
+        speed.append(sp)
     depth_a = np.stack(depth)
     init_s = float(np.sum(np.asarray(inputs.initial_state.h, np.float64) * area))
     init_n = float(np.sum(np.asarray(inputs.initial_state.node_volume, np.float64)))
@@ -160,7 +166,9 @@ def oracle_run(inputs, duration_s, snapshot_interval_s=3600.0, max_steps_per_hou
            + ledger["unreturned_overflow"] + ledger["recharge_loss"] - ledger["boundary_net_flux"])
     report = {"max_depth_m": float(np.max(depth_a)), **{f"{k}_m3": v for k, v in ledger.items()},
               "mass_residual_ratio": (lhs - rhs) / max(abs(lhs), 1.0), "step_count": clock.global_step}
-    return {"snapshot_times_s": np.asarray(times, np.float32), "depth_snapshots_m": depth_a, "report": report}
+    return {"snapshot_times_s": np.asarray(times, np.float32), "depth_snapshots_m": depth_a, "report": report,
+            "steps_at_snapshot": np.asarray(steps, np.int64), "speed_snapshots_mps": np.stack(speed),
+            "drain_node_volume_snapshots_m3": np.stack(nodes)}
 
 
 GOLDEN_DIR = __import__("pathlib").Path(__file__).resolve().parent / "golden"
@@ -197,3 +205,73 @@ def strip_network(inp, links=True, inlets=True):
         coupling = coupling._replace(**{f: np.asarray(v)[:0] for f, v in coupling._asdict().items()
                                         if f.startswith("inlet_")})
     return inp._replace(coupling=coupling, pipe=pipe, initial_state=state)
+
+
+def write_asset_tree(root, inp, with_multiplier=True):
+    """Writes ``inp`` as a V1 asset tree in the reference's schema (parity_assets.py,
+    parity_inputs.py) under ``root`` and returns (topology arrays, Horton arrays,
+    rainfall CSV path, an inactive cell)."""
+    import csv
+    from floodhip import assets
+    s, c, p = inp.surface, inp.coupling, inp.pipe
+    cen = np.asarray(inp.host["centroids"], np.float64)
+    C = s.bed.shape[0]
+    N = c.node_storage_area.shape[0]
+    topo = {
+        "nodes_world": np.asarray(inp.host["nodes"], np.float64), "triangles": np.asarray(inp.host["triangles"]),
+        "centroids_world": cen, "bed_elevation_m": s.bed, "manning_n": s.manning_n,
+        "active_surface_mask": s.active.astype(np.uint8), "surface_cell_area_m2": s.cell_area,
+        "surface_characteristic_length_m": s.characteristic_length, "surface_edge_left_cell": s.edge_left,
+        "surface_edge_right_cell": s.edge_right, "surface_edge_normal_x": s.edge_nx,
+        "surface_edge_normal_y": s.edge_ny, "surface_edge_length_m": s.edge_length,
+        "surface_boundary_code": s.boundary_code, "inlet_cell_index": c.inlet_cell, "inlet_node_index": c.inlet_node,
+        "node_surface_cell_index": c.node_surface_cell, "node_invert_elevation_m": c.node_invert,
+        "node_storage_area_m2": c.node_storage_area, "node_max_depth_m": c.node_max_depth,
+        "inlet_capture_capacity_m3ps": c.inlet_capture_capacity,
+        "inlet_surcharge_capacity_m3ps": c.inlet_surcharge_capacity, "surcharge_cell_index": c.surcharge_cell,
+        "surcharge_node_index": c.surcharge_node, "surcharge_return_weight": c.surcharge_weight,
+        "surcharge_return_weight_is_normalized": np.array([c.surcharge_weight_normalized]),
+        "link_from_node_index": p.link_from, "link_to_node_index": p.link_to, "link_length_m": p.link_length,
+        "link_area_m2": p.link_area, "link_hydraulic_radius_m": p.link_radius, "link_manning_n": p.link_manning,
+        "link_flow_capacity_m3ps": p.link_capacity, "link_invert_elevation_m": p.link_invert,
+        "link_diameter_m": p.link_diameter, "link_minor_loss_coefficient": p.link_minor_loss,
+        "link_geometry_code": p.link_geometry_code, "link_rect_width_m": p.link_rect_width,
+        "link_rect_height_m": p.link_rect_height, "outfall_area_m2": p.outfall_area,
+        "outfall_coefficient": p.outfall_coefficient, "outfall_tailwater_head_m": p.outfall_tailwater,
+        "outfall_flow_capacity_m3ps": p.outfall_capacity,
+        "node_initial_fill_depth_m": np.linspace(0.0, 0.5, N).astype(np.float32),
+    }
+    if with_multiplier:
+        topo["rainfall_multiplier"] = np.linspace(0.8, 1.2, C).astype(np.float32)
+    paths = assets.asset_paths(root)
+    for path in paths.values():
+        path.parent.mkdir(parents=True, exist_ok=True)
+    np.savez(paths["topology"], **topo)
+    rng = np.random.default_rng(0)
+    horton = {"horton_f0_mmhr": rng.uniform(5, 20, C).astype(np.float32),
+              "horton_fc_mmhr": rng.uniform(1, 5, C).astype(np.float32),
+              "horton_decay_per_hr": rng.uniform(1, 4, C).astype(np.float32)}
+    horton.update({k: np.zeros(C, np.float32) for k in ("building_fraction", "impervious_fraction",
+                                                         "bcr_open_fraction")})
+    np.savez(paths["horton"], **horton)
+    inactive = int(np.flatnonzero(~s.active)[0])
+    with paths["recharge"].open("w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["hydraulic_enabled", "source_cell_index", "x_m", "y_m", "storage_capacity_m3",
+                    "capture_capacity_m3ps", "recharge_capacity_m3ps", "stop_depth_m"])
+        w.writerow(["true", 5, 0, 0, 10.0, 0.05, 0.005, 0.03])
+        w.writerow(["no", 6, 0, 0, 99.0, 0.05, 0.005, 0.03])                 # disabled
+        w.writerow(["1", inactive, cen[inactive, 0], cen[inactive, 1], 20.0, 0.02, 0.001, ""])   # off the mesh
+    with paths["historical_boundary_inflow"].open("w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["inflow_id", "x_utm43n_m", "y_utm43n_m", "time_s", "q_m3ps"])
+        for t, q in ((0, 0.1), (600, 0.5), (1200, 0.2)):
+            w.writerow(["A", cen[10, 0], cen[10, 1], t, q])
+            w.writerow(["B", cen[40, 0], cen[40, 1], t, 2 * q])
+        w.writerow(["A", cen[10, 0], cen[10, 1], 600, 0.25])                 # same id, cell and time: summed
+    rain = root / "rain.csv"
+    with rain.open("w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["start_min", "end_min", "intensity_mmhr"])
+        w.writerows([(0, 30, 40.0), (30, 60, 90.0), (60, 120, 10.0)])
+    return topo, horton, rain, inactive

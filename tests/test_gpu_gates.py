@@ -37,3 +37,24 @@ def test_eight_gates_pass_against_oracle(case):
     assert res["passed"], res["gates"]
     assert cand["report"]["step_count"] == ref["report"]["step_count"]
     assert np.max(np.abs(cand["depth_snapshots_m"][-1] - ref["depth_snapshots_m"][-1])) <= 1e-3
+
+
+@pytest.mark.parametrize("mode", ["full_swe", "local_inertial"])
+def test_asset_tree_run_publishes_and_validates(tmp_path, mode):
+    """The real-asset path end to end: a V1 asset tree in the reference's schema ->
+    floodhip.assets.load_parity_inputs -> run_hip_v1_parity on the GPU -> publish,
+    validated like the reference validates (areas from the candidate report's
+    topology asset) against the oracle run of the same inputs."""
+    from floodhip import assets, runner
+    from helpers import small_network_scenario, write_asset_tree
+    inp = small_network_scenario(seed=7, h_init=0.15)
+    _, _, rain, _ = write_asset_tree(tmp_path / "v1", inp)
+    loaded = _wet(assets.load_parity_inputs(rainfall_csv=rain, v1_root=tmp_path / "v1", strict=False))
+    ref = oracle_run(loaded, 120.0, snapshot_interval_s=60.0, surface_mode=mode)
+    cand = run_hip_v1_parity(loaded, duration_s=120.0, snapshot_interval_s=60.0, surface_mode=mode)
+    assert cand["report"]["surface_step_mode"] == mode
+    assert cand["report"]["step_count"] == ref["report"]["step_count"]
+    ref_dir = runner.publish(ref, tmp_path / "ref", loaded)
+    cand_dir = runner.publish(cand, tmp_path / "cand", loaded)
+    res = gates.validate_run_dirs(ref_dir, cand_dir)
+    assert res["passed"], res["gates"]
