@@ -185,3 +185,18 @@ def test_lake_at_rest_stays_at_rest():
         got = s.get_state()
     assert np.array_equal(got.h, h0)
     assert not np.any(got.hu) and not np.any(got.hv)
+
+
+def test_compensated_is_closer_to_fp64():
+    """The compensated float32 local-inertial step (TwoSum term of h) stays within
+    float32 rounding of the plain run and no farther from the FP64 kernels."""
+    inp = _wet_bump_small(n=30)
+    area = inp.surface.cell_area.astype(np.float64)
+    out = {}
+    for key, kw in (("f64", dict(precision=64)), ("f32", dict()), ("comp", dict(compensated=True))):
+        with _solver(inp, fixed_dt=0.5, **kw) as s:
+            s.advance_to(200.0, 1e9, max_steps=400)
+            out[key] = s.get_state_f64()[0]
+    l2 = lambda a: float(np.sqrt(np.sum((a - out["f64"]) ** 2 * area) / area.sum()))  # noqa: E731
+    assert np.max(np.abs(out["comp"] - out["f32"])) <= 1e-4
+    assert l2(out["comp"]) <= l2(out["f32"]) * 1.05
