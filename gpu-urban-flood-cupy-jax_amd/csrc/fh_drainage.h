@@ -103,6 +103,11 @@ struct DrainDev {
     const float* ext_q_override;
     int use_override;
     const int *xc_cells, *xc_ptr, *xc_idx;
+    // dry-weather stepping (triangular_backend.py:833-952): no surface exchange, the
+    // outfall tailwater optionally overridden per node, overflow above capacity
+    // reported and (dry_cap) capped instead of returned
+    int dry, dry_cap;
+    const real* dry_tw;
 };
 
 template <typename real>
@@ -541,7 +546,8 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
     const int mem = d.seg_node.blk_member[b];
     const real zero = real(0);
     const int k0 = d.nl2_ptr[n], k1 = d.nl2_ptr[n + 1];
-    const bool entries = d.S > 0;
+    const bool dry = d.dry != 0;
+    const bool entries = d.S > 0 && !dry;
     const int e0 = entries ? d.ne_ptr[n] : 0, e1 = entries ? d.ne_ptr[n + 1] : 0;
     const real vp = d.Vp[n], f_self = d.sur_factor[n];
     const real inv = d.inv[n], As = d.As[n], dmax = d.dmax[n];
@@ -562,7 +568,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         fo[u] = d.sur_factor[ot[u]];
     }
     const real sw0 = entries ? d.nesw0[n] : zero;
-    const int rcell = d.nr_cell[n];
+    const int rcell = dry ? -1 : d.nr_cell[n];
     const real rA = d.nr_area[n];
     const real rh = state[rcell >= 0 ? rcell : 0].x;
     SlotAcc<3> acc3;
@@ -574,6 +580,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
     if (n < d.seg_node.mem_end[mem]) {
         const real dt = (real)ctl->dt_real;
         real vol = vp;
+        double moved = 0.0;    // sum of |transfer| over the links this node is 'from' of
         {
             real delta = zero;
             for (int kb = k0; kb < k1; kb += KB) {
@@ -597,6 +604,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
                     if (!is_to) {
                         delta = delta + (-T);
                         d.Q[en[u] >> 1] = T / vmax(dt, real(1.0e-12));
+                        moved += (double)fabs(T);
                     } else {
                         delta = delta + T;
                     }
@@ -609,15 +617,24 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         real outf = zero;
         if (flag >> 1) {
             const typename V4<real>::type o = d.outfall[(flag >> 1) - 1];
-            const real hd = head_of(inv, As, vol) - o.x;
+            const real hd = head_of(inv, As, vol) - (d.dry_tw ? d.dry_tw[n] : o.x);
             const real sgn = hd > zero ? real(1) : (hd < zero ? real(-1) : zero);
             real oq = o.y * o.z * sqrt(p.two_g * fabs(hd)) * sgn;
             oq = vmin(oq, vmax(o.w, zero));
             outf = vmin(vol, vmax(oq, zero) * dt);
             vol = vmax(vol - outf, zero);
         }
-        // overflow (v1_parity.py:371-404)
         const real capacity = As * vmax(dmax, zero);
+        if (dry) {
+            // dry weather: overflow reported, capped when asked, never returned
+            const real over = vmax(vol - capacity, zero);
+            if (d.dry_cap) vol = vmin(vol, capacity);
+            d.V[n] = vol;
+            out_sum = (double)outf;
+            total_sum = (double)over;
+            unret_sum = moved;
+        } else {
+        // overflow (v1_parity.py:371-404)
         const real over = vmax(vol - capacity, zero);
         real returned = zero;
         real wsum = zero;
@@ -649,6 +666,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         out_sum = (double)outf;
         total_sum = (double)(returned + fb);
         unret_sum = (double)vmax(vol - capacity, zero);
+        }
     }
     __shared__ double lds[48];
     double v[3] = {out_sum, total_sum, unret_sum};
@@ -729,6 +747,27 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
     double v[2] = {cap_sum, loss_sum};
     block_sum<2>(v, lds);
     acc2.commit(part, b, v);
+}
+
+// Dry-weather step prologue (triangular_backend.py:862-871): V = max(V, 0) +
+// max(inflow, 0) dt for every node, every inlet slot's exchange zeroed (no
+// surface exchange), and the member's step dt for the link and node kernels.
+template <typename real>
+__global__ void __launch_bounds__(256) k_dw_prep(Ctl* __restrict__ ctls, DrainDev<real> d, int n_slots,
+                                                 const real* __restrict__ inflow, double dt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const real dtr = (real)dt;
+    if (i < d.N) {
+        real v = vmax(d.V[i], real(0));
+        if (inflow) v = v + vmax(inflow[i], real(0)) * dtr;
+        d.V[i] = v;
+    }
+    if (i < n_slots) d.ni_net[i] = real(0);
+    if (i == 0) {
+        ctls[0].dt = (double)dtr;
+        ctls[0].dt_real = (double)dtr;
+        ctls[0].active = 1;
+    }
 }
 
 }  // namespace fh

@@ -336,6 +336,7 @@ public:
     virtual void get_ledger(double* l) = 0;
     virtual int advance_to(double target, double final_t, int64_t max_steps) = 0;
     virtual void step_once(double dt, double rain, const float* q, double* diag) = 0;
+    virtual void dry_weather_step(double dt, const float* inflow, const float* tw, int cap, double* diag) = 0;
     virtual double stable_timestep() = 0;
     virtual std::vector<KernelTiming> profile(int steps) = 0;
     virtual int64_t device_bytes() const = 0;
@@ -1380,6 +1381,7 @@ public:
             d_ni_cell.set(ni_cell, s); d_ni_surq.set(ni_surq, s);
             std::vector<real> zS((size_t)nslots, 0);
             d_ni_sur.set(zS, s); d_ni_net.set(zS, s);
+            n_inlet_slots = nslots;
             d_ic_node.set(ic_node, s); d_ic_slot.set(ic_slot, s); d_ic_capq.set(ic_capq, s);
             d_nl2_ptr.set(nl2_ptr, s); d_nl2_ent.set(nl2_ent, s); d_nl2_oth.set(nl2_oth, s);
             d_ne_sw.set(ne_sw, s);
@@ -1715,6 +1717,66 @@ public:
             for (int k = 0; k < fh::kLedgerTerms; ++k) diag[k] = ctl_h[cur].diag[k];
     }
 
+    void dry_weather_step(double dt, const float* inflow, const float* tw, int cap, double* diag) override {
+        ensure_packed();
+        if (M != 1) throw ValueError{"dry-weather stepping acts on single-scenario contexts"};
+        if (!(dt > 0.0)) throw ValueError{"timestep_s must be positive."};
+        const Member<real>& m = mem[0];
+        const real dtr = (real)dt;
+        double inflow_sum = 0.0;
+        if (inflow) {
+            std::vector<real> v((size_t)std::max(nodes.total, 1), real(0));
+            for (int64_t n = 0; n < m.N; ++n) {
+                const real q = (real)inflow[n];
+                v[(size_t)(nodes.base[0] + m.np((int)n))] = q;
+                const real qm = (q != q || q > real(0)) ? q : real(0);
+                inflow_sum += (double)(qm * dtr);
+            }
+            d_dw_in.set(v, stream);
+        }
+        if (tw) {
+            std::vector<real> v((size_t)std::max(nodes.total, 1), real(0));
+            for (int64_t n = 0; n < m.N; ++n) {
+                if (!std::isfinite(tw[n])) throw ValueError{"outfall tailwater heads must be finite"};
+                v[(size_t)(nodes.base[0] + m.np((int)n))] = (real)tw[n];
+            }
+            d_dw_tw.set(v, stream);
+        }
+        fold();
+        pull_ctl();
+        double saved[fh::kLedgerTerms];
+        std::memcpy(saved, ctl_h[0].ledger, sizeof(saved));
+        if (nodes.blocks > 0) {
+            const int threads = std::max(nodes.total, n_inlet_slots);
+            fh::k_dw_prep<real><<<(threads + 255) / 256, 256, 0, stream>>>(ctl, dd, n_inlet_slots,
+                                                                          inflow ? d_dw_in.p : nullptr, dt);
+            fh::DrainDev<real> d = dd;
+            d.dry = 1;
+            d.dry_cap = cap ? 1 : 0;
+            d.dry_tw = tw ? d_dw_tw.p : nullptr;
+            go_lds(fh::k_link_node<real>, nodes.blocks, fh::kLinkThreads, link_node_lds, ctl, kp, d, nodes.blocks,
+                   d_state.p);
+            go(fh::k_node_update<real>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
+        }
+        fold();
+        pull_ctl();
+        double final_volume = 0.0;
+        if (m.N > 0) {
+            std::vector<real> V((size_t)nodes.total);
+            d_V.get(V.data(), 0, (size_t)nodes.total, stream);
+            FH_CHECK(hipStreamSynchronize(stream));
+            for (int64_t n = 0; n < m.N; ++n) final_volume += (double)V[(size_t)(nodes.base[0] + m.np((int)n))];
+        }
+        // the node kernel's slots carry outfall, overflow and the moved volume in dry mode
+        diag[0] = inflow_sum;
+        diag[1] = ctl_h[0].diag[fh::L_UNRETURNED];
+        diag[2] = ctl_h[0].diag[fh::L_OUTFALL];
+        diag[3] = ctl_h[0].diag[fh::L_OVERFLOW];
+        diag[4] = final_volume;
+        std::memcpy(ctl_h[0].ledger, saved, sizeof(saved));
+        push_ctl();
+    }
+
     double stable_timestep() override {
         ensure_packed();
         launch_cfl();
@@ -1808,6 +1870,8 @@ public:
     DevArray<real> d_cell_n;          // local-inertial: Manning n per cell
     DevArray<unsigned char> d_perm;   // local-inertial: entry slots in ascending edge order
     DevArray<real> d_qli;             // local-inertial: edge discharge, for the donor pass
+    DevArray<real> d_dw_in, d_dw_tw;  // dry-weather step: node inflow and tailwater override
+    int n_inlet_slots = 0;
     bool li = false;                  // local-inertial surface mode
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
     DevArray<unsigned short> d_cls;   // source class per cell (when use_classes)
@@ -2064,6 +2128,14 @@ int fh_advance_to(fh_ctx* ctx, double target_s, double final_s, int64_t max_step
 
 int fh_step(fh_ctx* ctx, double dt, double rain_rate_mps, const float* external_q_m3ps, double* diag_out) {
     FH_GUARD(ctx, ctx->c.eng->step_once(dt, rain_rate_mps, external_q_m3ps, diag_out));
+    return FH_OK;
+}
+
+int fh_dry_weather_step(fh_ctx* ctx, double dt, const float* node_inflow_m3ps, const float* outfall_tailwater_head_m,
+                        int32_t cap_node_overflow, double* diag_out) {
+    if (!diag_out) return FH_ERR_VALUE;
+    FH_GUARD(ctx, ctx->c.eng->dry_weather_step(dt, node_inflow_m3ps, outfall_tailwater_head_m, cap_node_overflow,
+                                               diag_out));
     return FH_OK;
 }
 

@@ -75,6 +75,7 @@ SIGNATURES = {
     "fh_profile_kernels": (C.c_int, [_ctx, C.c_int32, C.c_int32, _i, C.c_char_p, _d, _d,
                                      C.POINTER(C.c_int64)]),
     "fh_device_bytes": (C.c_int64, [_ctx]),
+    "fh_dry_weather_step": (C.c_int, [_ctx, C.c_double, _f, _f, C.c_int32, _d]),
     "fh_copy_probe": (C.c_int, [C.c_int64, C.c_int32, _d]),
 }
 

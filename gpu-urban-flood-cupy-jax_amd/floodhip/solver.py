@@ -90,6 +90,15 @@ class SolverOptions(NamedTuple):
     max_speed: float = 5.0
 
 
+class PipeDryWeatherDiagnostics(NamedTuple):
+    """triangular_backend.py:233-240 (the scalar fields), in m^3."""
+    dry_weather_inflow_m3: float
+    link_abs_flow_m3: float
+    outfall_discharge_m3: float
+    overflow_surcharge_m3: float
+    final_node_volume_m3: float
+
+
 # config.py:157-158 of the reference: the surface operators of the step
 SURFACE_MODES = {"full_swe": 0, "local_inertial": 1}
 
@@ -349,6 +358,23 @@ class HipParitySolver:
         buf = (C.c_double * _lib.LEDGER_TERMS)()
         self._check(self.lib.fh_step(self._ctx, float(dt), float(rainfall_rate_mps), q, buf), "step")
         return MassLedger(*list(buf))
+
+    def dry_weather_step(self, dt, node_inflow_m3ps=None, outfall_tailwater_head_m=None,
+                         cap_node_overflow=True) -> "PipeDryWeatherDiagnostics":
+        """step_triangular_pipe_dry_weather_network (triangular_backend.py:833-952) on the
+        resident network: sewer-only flow with optional per-node inflow and tailwater
+        override; no surface exchange, clock and run ledger unchanged.  The node
+        volumes and link flows are read back with get_state()."""
+        if not float(dt) > 0.0:
+            raise ValueError("timestep_s must be positive.")
+        a = _Args()
+        inflow = None if node_inflow_m3ps is None else a.f(np.asarray(node_inflow_m3ps, np.float32).reshape(self.n_nodes))
+        tw = None if outfall_tailwater_head_m is None else a.f(
+            np.asarray(outfall_tailwater_head_m, np.float32).reshape(self.n_nodes))
+        buf = (C.c_double * 5)()
+        self._check(self.lib.fh_dry_weather_step(self._ctx, float(dt), inflow, tw, int(bool(cap_node_overflow)), buf),
+                    "dry_weather_step")
+        return PipeDryWeatherDiagnostics(*list(buf))
 
     def stable_timestep(self) -> float:
         out = C.c_double()

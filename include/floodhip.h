@@ -178,6 +178,18 @@ int fh_step(fh_ctx* ctx, double dt, double rain_rate_mps, const float* external_
 /* stable_timestep (v1_parity.py:153-158) on the resident state. */
 int fh_stable_timestep(fh_ctx* ctx, double* dt_out);
 
+/* Sewer-only dry-weather flow, step_triangular_pipe_dry_weather_network
+ * (gurugram_flood/kernels/triangular_backend.py:833-952): node inflow (m^3/s per
+ * node in the caller's order, NULL = none), the link update and outfalls of the
+ * coupled step, the overflow above storage capacity reported and capped when
+ * cap_node_overflow != 0.  No surface exchange; the clock and the run ledger are
+ * unchanged.  outfall_tailwater_head_m (finite, per node, NULL = the pipe
+ * table's) overrides the tailwater.  diag_out gets 5 values: dry-weather inflow,
+ * sum of |link transfer|, outfall discharge, overflow, final node volume (m^3).
+ * Single-scenario contexts only. */
+int fh_dry_weather_step(fh_ctx* ctx, double dt, const float* node_inflow_m3ps,
+                        const float* outfall_tailwater_head_m, int32_t cap_node_overflow, double* diag_out);
+
 /* Measurement: runs `steps` steps of the plain step sequence (no graph), timing
  * each kernel with HIP events on the context stream.  names/ms/bytes are
  * arrays of length cap; returns the number of kernels reported in *count.

@@ -389,14 +389,17 @@ def _circular_radius(depth, diameter, dtype):
     return _circular_area(depth, diameter, dtype) / np.maximum(perimeter, F(1.0e-12))
 
 
-def pipe_links_and_outfalls(state, coupling, pipe, dt, dtype=np.float32, slot_wave_speed=30.0):
-    """v1_parity.py:324-368 -- semi-implicit link momentum, donor-limited transfer, outfalls."""
+def pipe_links_and_outfalls(state, coupling, pipe, dt, dtype=np.float32, slot_wave_speed=30.0,
+                            with_transfer=False):
+    """v1_parity.py:324-368 -- semi-implicit link momentum, donor-limited transfer, outfalls.
+    ``with_transfer`` also returns the float64 sum of |transfer| over the links."""
     F = _f(dtype)
     dt = F(dt)
     g = F(GRAVITY)
     n_nodes = state.node_volume.shape[0]
+    transfer_abs = 0.0
     if pipe.link_from.size == 0 and n_nodes == 0:
-        return state, 0.0
+        return (state, 0.0, 0.0) if with_transfer else (state, 0.0)
     head = _node_head(state.node_volume, coupling, dtype)
     source, target = pipe.link_from, pipe.link_to
     node_volume = state.node_volume
@@ -437,6 +440,7 @@ def pipe_links_and_outfalls(state, coupling, pipe, dt, dtype=np.float32, slot_wa
                                  np.concatenate((-transfer, transfer)), n_nodes)
         node_volume = node_volume + node_delta
         link_flow = transfer / np.maximum(dt, F(1.0e-12))
+        transfer_abs = float(np.sum(np.abs(transfer).astype(np.float64)))
     node_volume = np.maximum(node_volume, F(0))     # applied to every node, links or not
     head = _node_head(node_volume, coupling, dtype)
     dh = head - pipe.outfall_tailwater
@@ -444,8 +448,44 @@ def pipe_links_and_outfalls(state, coupling, pipe, dt, dtype=np.float32, slot_wa
     outfall_q = np.minimum(outfall_q, np.maximum(pipe.outfall_capacity, F(0)))
     outfall = np.minimum(node_volume, np.maximum(outfall_q, F(0)) * dt)
     node_volume = np.maximum(node_volume - outfall, F(0))
-    return (state._replace(node_volume=node_volume.astype(dtype), link_flow=link_flow.astype(dtype)),
-            float(np.sum(outfall.astype(np.float64))))
+    out = (state._replace(node_volume=node_volume.astype(dtype), link_flow=link_flow.astype(dtype)),
+           float(np.sum(outfall.astype(np.float64))))
+    return out + (transfer_abs,) if with_transfer else out
+
+
+DRY_WEATHER_FIELDS = ("dry_weather_inflow", "link_abs_flow", "outfall_discharge", "overflow_surcharge",
+                      "final_node_volume")
+
+
+def dry_weather_step(state, coupling, pipe, dt, node_inflow_m3ps=None, outfall_tailwater_head_m=None,
+                     cap_node_overflow=True, dtype=np.float32):
+    """gurugram_flood/kernels/triangular_backend.py:833-952 -- sewer-only dry-weather flow:
+    node inflow, then the link momentum / donor-limited transfer / outfall update of
+    v1_parity.py:324-368 (the same physics as the reference's dynamic pipe graph), then
+    the overflow above storage capacity reported and, by default, capped.  No surface
+    exchange.  Returns (state, diagnostics over DRY_WEATHER_FIELDS, float64)."""
+    F = _f(dtype)
+    dt = F(dt)
+    volume = np.maximum(state.node_volume, F(0))
+    if node_inflow_m3ps is None:
+        inflow = volume * F(0)
+    else:
+        inflow = np.maximum(np.asarray(node_inflow_m3ps, dtype), F(0))
+    inflow_m3 = inflow * dt
+    volume = volume + inflow_m3
+    if outfall_tailwater_head_m is not None:
+        pipe = pipe._replace(outfall_tailwater=np.asarray(outfall_tailwater_head_m, dtype))
+    st, outfall, transfer_abs = pipe_links_and_outfalls(state._replace(node_volume=volume), coupling, pipe, dt,
+                                                        dtype, with_transfer=True)
+    volume = st.node_volume
+    capacity = coupling.node_storage_area * np.maximum(coupling.node_max_depth, F(0))
+    overflow = np.maximum(volume - capacity, F(0))
+    if cap_node_overflow:
+        volume = np.minimum(volume, capacity)
+    diag = dict(zip(DRY_WEATHER_FIELDS, (float(np.sum(inflow_m3.astype(np.float64))), transfer_abs, outfall,
+                                         float(np.sum(overflow.astype(np.float64))),
+                                         float(np.sum(volume.astype(np.float64))))))
+    return st._replace(node_volume=volume.astype(dtype)), diag
 
 
 def return_node_overflow(state, surface, coupling, dtype=np.float32):

@@ -152,3 +152,38 @@ def test_degenerate_networks_match_oracle(links, inlets):
             diag = s.step(dt, rate, q)
             t += dt
             _compare(s.get_state(), st, diag, rdiag)
+
+
+def test_dry_weather_steps_match_oracle_and_leave_the_run_untouched():
+    """step_triangular_pipe_dry_weather_network (triangular_backend.py:833-952) on the
+    GPU against the oracle's restatement: node volumes and link flows 1e-4 relative,
+    the five diagnostics 1e-4 relative; the surface, the clock and the run ledger are
+    unchanged, and a coupled step afterwards still matches the oracle."""
+    inp = small_network_scenario(seed=7, h_init=0.3)
+    io = to_oracle(inp)
+    s_, c_, p_, r_, st, f_ = io
+    N = c_.node_storage_area.shape[0]
+    rng = np.random.default_rng(5)
+    inflow = rng.uniform(-0.005, 0.03, N).astype(np.float32)
+    tw = (np.asarray(p_.outfall_tailwater) + 0.05).astype(np.float32)
+    with _solver(inp) as s:
+        before = s.get_state()
+        clock0, ledger0 = s.clock(), np.asarray(tuple(s.ledger()))
+        for i, (q, t, cap) in enumerate(((inflow, None, True), (inflow, tw, True), (None, None, False))):
+            st, rd = orc.dry_weather_step(st, c_, p_, 20.0, q, t, cap)
+            d = s.dry_weather_step(20.0, q, t, cap)
+            got = s.get_state()
+            vs = np.maximum(np.abs(st.node_volume), 1.0)
+            assert np.max(np.abs(got.node_volume - st.node_volume) / vs) <= 1e-4, i
+            qs = np.maximum(np.abs(st.link_flow), 1e-3)
+            assert np.max(np.abs(got.link_flow - st.link_flow) / qs) <= 1e-4, i
+            for k, v in zip(orc.DRY_WEATHER_FIELDS, d):
+                assert abs(v - rd[k]) <= 1e-4 * max(abs(rd[k]), 1e-3), (i, k, v, rd[k])
+        assert np.array_equal(got.h, before.h) and np.array_equal(got.hu, before.hu)
+        assert s.clock() == clock0
+        assert np.array_equal(np.asarray(tuple(s.ledger())), ledger0)
+        rate = orc.rain_rate(f_, 0.0)
+        q = orc.external_q(f_, 0.2)
+        ref, rdiag = oracle_step(io, st, 0.4, rain=rate, q=q)
+        diag = s.step(0.4, rate, q)
+        _compare(s.get_state(), ref, diag, rdiag)

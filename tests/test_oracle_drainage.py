@@ -149,3 +149,47 @@ def test_rain_and_external_lookup():
     assert np.allclose(orc.external_q(f, 15.0), q[0] + 0.5 * (q[1] - q[0]), rtol=1e-6)
     assert np.allclose(orc.external_q(f, -5.0), q[0])
     assert np.allclose(orc.external_q(f, 1e6), q[-1])
+
+
+def _network64(seed=7):
+    from helpers import small_network_scenario, to_oracle
+    inp = small_network_scenario(seed=seed, h_init=0.2)
+    s, c, p, r, st, f = to_oracle(inp, F64)
+    return c, p, st
+
+
+def test_dry_weather_balance_with_outfalls_and_cap():
+    """triangular_backend.py:833-952: V_after = V_before + inflow - outfall - capped
+    overflow, every node at or below its capacity, diagnostics consistent."""
+    c, p, st = _network64()
+    rng = np.random.default_rng(1)
+    inflow = rng.uniform(-0.01, 0.05, c.node_storage_area.shape[0])
+    v0 = float(np.sum(np.maximum(st.node_volume, 0)))
+    new, d = orc.dry_weather_step(st, c, p, 30.0, inflow, dtype=F64)
+    assert d["dry_weather_inflow"] == pytest.approx(float(np.sum(np.maximum(inflow, 0) * 30.0)), rel=1e-14)
+    assert d["outfall_discharge"] > 0 and d["link_abs_flow"] > 0 and d["overflow_surcharge"] > 0
+    balance = v0 + d["dry_weather_inflow"] - d["outfall_discharge"] - d["overflow_surcharge"]
+    assert float(np.sum(new.node_volume)) == pytest.approx(balance, rel=1e-12)
+    assert d["final_node_volume"] == pytest.approx(float(np.sum(new.node_volume)), rel=1e-14)
+    cap = c.node_storage_area * np.maximum(c.node_max_depth, 0)
+    assert np.all(new.node_volume <= cap)
+
+
+def test_dry_weather_closed_outfalls_conserve_and_uncapped_overflow_stays():
+    c, p, st = _network64()
+    p = p._replace(outfall_area=np.zeros_like(p.outfall_area))
+    inflow = np.full(c.node_storage_area.shape[0], 0.02)
+    v0 = float(np.sum(np.maximum(st.node_volume, 0)))
+    new, d = orc.dry_weather_step(st, c, p, 60.0, inflow, cap_node_overflow=False, dtype=F64)
+    assert d["outfall_discharge"] == 0.0
+    assert float(np.sum(new.node_volume)) == pytest.approx(v0 + d["dry_weather_inflow"], rel=1e-12)
+    cap = c.node_storage_area * np.maximum(c.node_max_depth, 0)
+    assert d["overflow_surcharge"] == pytest.approx(float(np.sum(np.maximum(new.node_volume - cap, 0))), rel=1e-12)
+    assert np.any(new.node_volume > cap)
+
+
+def test_dry_weather_tailwater_above_heads_stops_outfalls():
+    c, p, st = _network64()
+    high = np.full(c.node_storage_area.shape[0], 1.0e4)
+    _, d = orc.dry_weather_step(st, c, p, 30.0, None, outfall_tailwater_head_m=high, dtype=F64)
+    assert d["outfall_discharge"] == 0.0 and d["dry_weather_inflow"] == 0.0
