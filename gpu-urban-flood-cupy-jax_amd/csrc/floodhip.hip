@@ -1744,8 +1744,7 @@ public:
         }
         fold();
         pull_ctl();
-        double saved[fh::kLedgerTerms];
-        std::memcpy(saved, ctl_h[0].ledger, sizeof(saved));
+        const fh::Ctl saved = ctl_h[0];     // the run's dt, flags and ledger, restored below
         if (nodes.blocks > 0) {
             const int threads = std::max(nodes.total, n_inlet_slots);
             fh::k_dw_prep<real><<<(threads + 255) / 256, 256, 0, stream>>>(ctl, dd, n_inlet_slots,
@@ -1773,7 +1772,11 @@ public:
         diag[2] = ctl_h[0].diag[fh::L_OUTFALL];
         diag[3] = ctl_h[0].diag[fh::L_OVERFLOW];
         diag[4] = final_volume;
-        std::memcpy(ctl_h[0].ledger, saved, sizeof(saved));
+        std::memcpy(ctl_h[0].ledger, saved.ledger, sizeof(saved.ledger));
+        std::memcpy(ctl_h[0].diag, saved.diag, sizeof(saved.diag));
+        ctl_h[0].dt = saved.dt;
+        ctl_h[0].dt_real = saved.dt_real;
+        ctl_h[0].active = saved.active;
         push_ctl();
     }
 
