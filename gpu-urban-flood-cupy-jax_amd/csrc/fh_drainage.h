@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
                     if (!is_to) {
                         delta = delta + (-T);
                         d.Q[en[u] >> 1] = T / vmax(dt, real(1.0e-12));
-                        moved += (double)fabs(T);
+                        if (dry) moved += (double)fabs(T);
                     } else {
                         delta = delta + T;
                     }
