@@ -135,3 +135,23 @@ def test_failure_modes_raise_reference_errors():
         s.set_state(inp.initial_state._replace(h=h))
         with pytest.raises(SimulationError):
             s.advance_to(s.clock()["time_s"] + 1.0, 1000.0)
+
+
+def test_mode_and_dry_weather_argument_errors():
+    """Unknown surface modes, a dry-weather step on a batch, a non-positive dt and a
+    non-finite tailwater override raise ValueError; the context stays usable."""
+    from floodhip.solver import HipEnsemble
+    inp = small_network_scenario(seed=4, h_init=0.2)
+    with pytest.raises(ValueError):
+        _solver(inp, surface_mode="diffusive")
+    with _solver(inp) as s:
+        n = inp.coupling.node_storage_area.shape[0]
+        with pytest.raises(ValueError):
+            s.dry_weather_step(0.0)
+        with pytest.raises(ValueError):
+            s.dry_weather_step(1.0, outfall_tailwater_head_m=np.full(n, np.nan, np.float32))
+        d = s.dry_weather_step(1.0, np.full(n, 0.01, np.float32))
+        assert d.dry_weather_inflow_m3 == pytest.approx(0.01 * n, rel=1e-6)
+    with HipEnsemble([inp, inp]) as ens:
+        with pytest.raises(ValueError):
+            ens.dry_weather_step(1.0)
