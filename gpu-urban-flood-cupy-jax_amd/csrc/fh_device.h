@@ -67,6 +67,7 @@ struct Clock {
 struct Ctl {
     Clock clk[2];
     unsigned long long cfl_bits[2];   // running CFL minimum of the step of that parity
+    int64_t li_cfl_step[2];           // step whose local-inertial CFL pass filled that slot (-1: none)
     double target;         // stop when t >= target
     double final_t;        // event end time (dt cap final_t - t)
     int64_t max_steps;     // step budget of the current advance call
@@ -83,7 +84,7 @@ struct Ctl {
     double ext_frac;       // external inflow interpolation fraction at t + dt/2
     int32_t ext_lower, ext_upper;
     int32_t nonfinite;     // count of non-finite values found by k_finite
-    int32_t pad0;
+    int32_t li_missed;     // a local-inertial step evaluated the CFL condition without its CFL pass
     double cfl_min;        // result of the last CFL reduction
     double diag[kLedgerTerms];    // this step's ledger increments
     double ledger[kLedgerTerms];  // cumulative FP64 ledger

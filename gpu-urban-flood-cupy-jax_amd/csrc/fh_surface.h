@@ -283,6 +283,7 @@ __device__ __forceinline__ void commit_plan(Ctl* ctl, int parity, const Clock& c
     }
     ctl->clk[1 - parity] = n;
     ctl->cfl_bits[1 - parity] = (unsigned long long)__double_as_longlong(INFINITY);
+    ctl->li_cfl_step[1 - parity] = -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -452,6 +453,7 @@ __global__ void __launch_bounds__(256) k_li_cfl(Ctl* __restrict__ ctls, KP<real>
         if (local < mn) mn = local;
     }
     commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
+    if (b == blk0 && threadIdx.x == 0) ctl->li_cfl_step[parity] = ck.step;
 }
 
 // One edge's local-inertial discharge (triangular_backend.py:1196-1248 with
@@ -482,7 +484,11 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
     const bool eval = ck.active && ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);
     const double gmin = eval ? __longlong_as_double((long long)ctl->cfl_bits[parity]) : INFINITY;
     const StepPlan<real> pl = plan_step<real>(ctl, ck, p, ft, m, gmin);
-    if (!pl.active || e >= end) return;
+    if (!pl.active) return;
+    // an evaluating step whose CFL pass did not run (a graph captured for another
+    // step phase) would take dt from an empty minimum: flag it for the host
+    if (eval && threadIdx.x == 0 && ctl->li_cfl_step[parity] != ck.step) ctl->li_missed = 1;
+    if (e >= end) return;
     const real zero = real(0), half = real(0.5);
     const bool interior = r >= 0;
     const bool outlet = r == kGhostOutflow || r == kGhostCopy;

@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <stdexcept>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -361,6 +362,7 @@ public:
                 ck.eval = P.fixed_dt > 0.0 ? 0 : 1;
                 ck.rain.next_end = -INFINITY;
                 std::memcpy(&m.ctl.cfl_bits[q], &inf, sizeof(inf));
+                m.ctl.li_cfl_step[q] = -1;
             }
         }
         for (int m = 0; m < M; ++m) ctl_h[m] = mem[m].ctl;
@@ -1555,9 +1557,11 @@ public:
 
     // One step: flux and update on the clock of parity `par`; the update writes
     // the next step's clock into the other slot, so the parity flips per step.
-    void launch_step() {
+    // cfl_pass: the local-inertial CFL pass is launched (a single-scenario graph
+    // captured for its step phase leaves it out of the steps that cannot evaluate)
+    void launch_step(bool cfl_pass = true) {
         if (li) {
-            launch_li_cfl();
+            if (cfl_pass) launch_li_cfl();
             launch_li_flux();
             launch_li_donor();
         } else {
@@ -1576,23 +1580,33 @@ public:
         int steps = 0;
     };
     StepGraph graphs[2][2];   // [full / short][parity of the first step]
+    // single-scenario local-inertial graphs, keyed by (full / short, parity, step
+    // phase mod the CFL interval): only the steps that can evaluate launch the CFL pass
+    std::map<int64_t, StepGraph> li_graphs;
+    int64_t host_step = 0;    // the member's step count at the next chunk (phase-keyed graphs)
 
-    void drop_graph() {
-        for (auto& row : graphs)
-            for (auto& g : row) {
-                if (g.exec) (void)hipGraphExecDestroy(g.exec);
-                if (g.graph) (void)hipGraphDestroy(g.graph);
-                g = StepGraph{};
-            }
-    }
+    bool li_phased() const { return li && M == 1 && P.use_graph; }
 
-    void build_graph(StepGraph& g, int k) {
+    static void destroy(StepGraph& g) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
         if (g.graph) (void)hipGraphDestroy(g.graph);
         g = StepGraph{};
+    }
+
+    void drop_graph() {
+        for (auto& row : graphs)
+            for (auto& g : row) destroy(g);
+        for (auto& kv : li_graphs) destroy(kv.second);
+        li_graphs.clear();
+    }
+
+    // phase < 0: every step launches the CFL pass
+    void build_graph(StepGraph& g, int k, int64_t phase = -1) {
+        destroy(g);
         const int p0 = par;
         FH_CHECK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < k; ++i) launch_step();
+        for (int i = 0; i < k; ++i)
+            launch_step(phase < 0 || (kp.fixed_dt <= 0.0 && (phase + i) % kp.interval == 0));
         FH_CHECK(hipStreamEndCapture(stream, &g.graph));
         par = p0;   // capture only recorded the steps
         FH_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
@@ -1600,7 +1614,18 @@ public:
     }
 
     void launch_chunk(int k) {
-        if (P.use_graph) {
+        if (li_phased()) {
+            // steps past the target are no-ops and do not count, so host_step is exact
+            // for every step that runs
+            const int64_t phase = host_step % kp.interval;
+            const int64_t key = ((int64_t)(k == std::max(1, P.steps_per_launch) ? 0 : 1) * 2 + par) *
+                                    (int64_t)kp.interval + phase;
+            StepGraph& g = li_graphs[key];
+            if (!g.exec || g.steps != k) build_graph(g, k, phase);
+            FH_CHECK(hipGraphLaunch(g.exec, stream));
+            par ^= (k & 1);
+            host_step += k;
+        } else if (P.use_graph) {
             StepGraph& g = graphs[k == std::max(1, P.steps_per_launch) ? 0 : 1][par];
             if (!g.exec || g.steps != k) build_graph(g, k);
             FH_CHECK(hipGraphLaunch(g.exec, stream));
@@ -1652,6 +1677,7 @@ public:
         const int K = std::max(1, P.steps_per_launch);
         pull_ctl();
         while (any_running(target, max_steps)) {
+            host_step = ctl_h[0].clk[par].step;
             // several chunks when every member is far from the target (extra steps are no-ops)
             double min_rem = 1e300;
             int64_t budget_left = 0;
@@ -1680,6 +1706,9 @@ public:
         fold();
         finalize_pending();
         const bool finite = check_finite();
+        for (int m = 0; m < M; ++m)
+            if (ctl_h[m].li_missed) throw std::runtime_error("local-inertial step evaluated the CFL condition "
+                                                             "without its CFL pass");
         if (!finite) return FH_ERR_SIMULATION;
         for (int m = 0; m < M; ++m)
             if (ctl_h[m].clk[par].t < target) return FH_ERR_BUDGET;
