@@ -1,0 +1,104 @@
+"""Full-scale parity check on the production workload (981,880 triangles, coupled
+118,768-node network): the CPU oracle against the HIP path, in either surface mode.
+Not collected by pytest: the oracle half takes minutes of CPU time.
+
+    python tests/fullscale_parity.py oracle full_swe OUT.npz [T_END]   # CPU: oracle run, CFL policy
+    python tests/fullscale_parity.py gpu    full_swe OUT.npz [T_END]   # GPU: same run, prints the comparison
+    python tests/fullscale_parity.py oracle-steps OUT.npz              # CPU: 20 fixed-dt steps, node volumes per step
+    python tests/fullscale_parity.py gpu-steps OUT.npz                 # GPU: per-step node-volume comparison
+
+The comparison reports step counts, the largest depth difference, the depth RMSE
+over active cells, surface and node volume differences and the ledger terms.
+"""
+
+import json
+import sys
+import time
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+sys.path[:0] = [str(HERE), str(HERE.parent), str(HERE.parent / "gpu-urban-flood-cupy-jax_amd")]
+
+import numpy as np  # noqa: E402
+
+from floodhip import synth  # noqa: E402
+from helpers import oracle_step, to_oracle  # noqa: E402
+from oracle import v1_parity_np as orc  # noqa: E402
+
+LEDGER_KEYS = ("rainfall_input", "infiltration_loss", "boundary_outflow", "surface_capture", "outfall_discharge",
+               "overflow_surcharge")
+
+
+def run_oracle(mode, out, t_end):
+    inp = synth.make_production_scale()
+    s, c, p, r, st, f = to_oracle(inp)
+    t0 = time.time()
+    st, clock, led = orc.advance_to_time(st, orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger(), t_end, 3600.0,
+                                         s, c, p, r, f, max_steps=100_000, surface_mode=mode)
+    print(f"oracle {mode}: {clock.global_step} steps to t={clock.time_s:.3f}s in {time.time() - t0:.1f}s", flush=True)
+    np.savez_compressed(out, h=st.h, node_volume=st.node_volume, steps=clock.global_step, time_s=clock.time_s,
+                        **{f"ledger_{k}": v for k, v in led.items()})
+
+
+def run_gpu(mode, out, t_end):
+    from floodhip.solver import HipParitySolver
+    inp = synth.make_production_scale()
+    ref = np.load(out)
+    with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
+                         surface_mode=mode) as s:
+        clock, ledger = s.advance_to(t_end, 3600.0, max_steps=100_000)
+        got = s.get_state()
+    area = inp.surface.cell_area.astype(np.float64)
+    dh = got.h.astype(np.float64) - ref["h"]
+    vol_ref = float(np.sum(ref["h"] * area))
+    rel_v = np.abs(got.node_volume - ref["node_volume"]) / np.maximum(np.abs(ref["node_volume"]), 1.0)
+    print(json.dumps({
+        "mode": mode, "steps": [int(clock["global_step"]), int(ref["steps"])],
+        "max_abs_dh_m": float(np.max(np.abs(dh))), "rmse_active_m": float(np.sqrt(np.mean(dh[inp.surface.active] ** 2))),
+        "cells_dh_gt_1mm": int(np.sum(np.abs(dh) > 1e-3)),
+        "surface_volume_rel": (float(np.sum(got.h * area)) - vol_ref) / vol_ref,
+        "node_volume_rel_max": float(rel_v.max()), "nodes_rel_gt_1e-3": int(np.sum(rel_v > 1e-3)),
+        "ledger_rel": {k: (getattr(ledger, k) - float(ref[f"ledger_{k}"])) / max(abs(float(ref[f"ledger_{k}"])), 1e-3)
+                       for k in LEDGER_KEYS}}), flush=True)
+
+
+def _schedule(io, steps, dt):
+    return [(orc.rain_rate(io[5], i * dt), orc.external_q(io[5], i * dt + 0.5 * dt)) for i in range(steps)]
+
+
+def run_oracle_steps(out, steps=20, dt=0.5):
+    inp = synth.make_production_scale()
+    io = to_oracle(inp)
+    st = io[4]
+    vols = []
+    for rate, q in _schedule(io, steps, dt):
+        st, _ = oracle_step(io, st, dt, rain=rate, q=q)
+        vols.append(st.node_volume.copy())
+    np.savez_compressed(out, vols=np.stack(vols), h=st.h)
+
+
+def run_gpu_steps(out, steps=20, dt=0.5):
+    from floodhip.solver import HipParitySolver
+    inp = synth.make_production_scale()
+    io = to_oracle(inp)
+    ref = np.load(out)
+    with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state) as s:
+        for i, (rate, q) in enumerate(_schedule(io, steps, dt)):
+            s.step(dt, rate, q)
+            got = s.get_state()
+            rel = np.abs(got.node_volume - ref["vols"][i]) / np.maximum(np.abs(ref["vols"][i]), 1.0)
+            print(json.dumps({"step": i + 1, "node_volume_rel_max": float(rel.max())}), flush=True)
+        print(json.dumps({"max_abs_dh_m": float(np.max(np.abs(got.h - ref["h"])))}), flush=True)
+
+
+if __name__ == "__main__":
+    cmd = sys.argv[1]
+    if cmd in ("oracle", "gpu"):
+        t_end = float(sys.argv[4]) if len(sys.argv) > 4 else 300.0
+        (run_oracle if cmd == "oracle" else run_gpu)(sys.argv[2], sys.argv[3], t_end)
+    elif cmd == "oracle-steps":
+        run_oracle_steps(sys.argv[2])
+    elif cmd == "gpu-steps":
+        run_gpu_steps(sys.argv[2])
+    else:
+        raise SystemExit(__doc__)
