@@ -99,7 +99,30 @@ struct Seg {
     const int* mem_blk0;     // [members]  first block of the member
     const int* mem_nblk;     // [members]  number of blocks of the member
     const int* mem_end;      // [members]  exclusive end of the member's real items
+    // Shared-mesh batch (sh_members > 0): every member has the same mesh, whose
+    // tables are stored once with member-local indices.  Member m's items are
+    // [m * sh_items, (m + 1) * sh_items) (sh_nblk blocks each), and sh_link is the
+    // per-member stride of the space this one indexes (cells for edges, edges
+    // for cells).  sh_recip = floor(2^40 / sh_nblk) + 1 divides a block index by
+    // sh_nblk exactly (the host checks blocks * sh_nblk < 2^40); sh_rot staggers
+    // the members' tile orders.
+    int sh_members = 0, sh_nblk = 0, sh_items = 0, sh_link = 0, sh_rot = 0;
+    unsigned long long sh_recip = 0;
 };
+
+// Block placement of a shared-mesh batch.  Members run one after another (slot
+// i belongs to member m = i / sh_nblk), and member m starts its tiles at
+// m * sh_rot (mod sh_nblk), so the XCDs, each working a contiguous range of
+// slots, read different parts of the shared tables at any moment instead of
+// all requesting the same lines.  Returns the block b = m * sh_nblk + t, whose
+// items, partial slots and member are those of the concatenated layout.  All
+// of it is scalar arithmetic on the block index.
+__device__ __forceinline__ int shared_block(const Seg& s, int i, int& m) {
+    m = (int)(((unsigned long long)(unsigned)i * s.sh_recip) >> 40);
+    int t = i - m * s.sh_nblk + m * s.sh_rot;
+    if (t >= s.sh_nblk) t -= s.sh_nblk;
+    return m * s.sh_nblk + t;
+}
 
 template <typename real>
 struct KP {                 // kernel constants, pre-rounded to the state precision

@@ -339,8 +339,10 @@ struct EdgeSeg {
 // a step that evaluates the CFL condition it also reduces the CFL time of both
 // cells of every edge (and of the orphan cells) into the step's minimum slot:
 // every cell counts at least once, and a cell counted twice gives the same value.
+// SH: shared-mesh batch; the edge tables hold member-local cells and are indexed
+// by the member-local edge.
 // ---------------------------------------------------------------------------
-template <typename real, int EPT>
+template <typename real, int EPT, bool SH>
 __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
         const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
@@ -364,8 +366,16 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
         return;
     }
-    const int b = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
-    const int m = es.seg.blk_member[b];
+    const int i = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
+    int b, m, g0 = 0, cb = 0;
+    if constexpr (SH) {
+        b = shared_block(es.seg, i, m);
+        g0 = m * es.seg.sh_items;
+        cb = m * es.seg.sh_link;
+    } else {
+        b = i;
+        m = es.seg.blk_member[b];
+    }
     // every load ahead of the control block: the grid covers the padded edge
     // space exactly, padding edges point at cell 0 as copy ghosts
     int lft[EPT], rgt[EPT];
@@ -373,10 +383,11 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
     V sls[EPT], srs[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
-        const int e = (b * EPT + k) * (int)blockDim.x + threadIdx.x;
-        lft[k] = el[e];
-        rgt[k] = er[e];
-        nxs[k] = enx[e]; nys[k] = eny[e]; lens[k] = elen[e];
+        const int g = (b * EPT + k) * (int)blockDim.x + threadIdx.x - g0;
+        lft[k] = el[g] + cb;
+        const int r = er[g];
+        rgt[k] = SH && r >= 0 ? r + cb : r;
+        nxs[k] = enx[g]; nys[k] = eny[g]; lens[k] = elen[g];
     }
 #pragma unroll
     for (int k = 0; k < EPT; ++k) {
@@ -462,19 +473,30 @@ __global__ void __launch_bounds__(256) k_li_cfl(Ctl* __restrict__ ctls, KP<real>
 // outflow-only at routable boundary edges (codes 4 and 1), zero at walls
 // (codes 0, 2, 3, inactive neighbours).  Writes rec[e] = {Q, Q nx, Q ny, 0} for
 // the update and qv[e] = Q, which the donor pass gathers (4 bytes, not 16).
-template <typename real>
+// SH as in k_flux.
+template <typename real, bool SH>
 __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
         ForcingTables ft, const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
         const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc,
         const real* __restrict__ man, typename V4<real>::type* __restrict__ rec, real* __restrict__ qv) {
     typedef typename V4<real>::type V;
-    const int b = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
-    const int m = es.seg.blk_member[b];
+    const int i = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
+    int b, m, g0 = 0, cb = 0;
+    if constexpr (SH) {
+        b = shared_block(es.seg, i, m);
+        g0 = m * es.seg.sh_items;
+        cb = m * es.seg.sh_link;
+    } else {
+        b = i;
+        m = es.seg.blk_member[b];
+    }
     const int e = b * (int)blockDim.x + threadIdx.x;
-    const int l = el[e], r = er[e];
+    const int g = e - g0;
+    const int l = el[g] + cb, r0 = er[g];
+    const int r = SH && r0 >= 0 ? r0 + cb : r0;
     const int rs = r >= 0 ? r : l;
-    const real nx = enx[e], ny = eny[e], len = elen[e];
+    const real nx = enx[g], ny = eny[g], len = elen[g];
     const V sl = state[l], sr = state[rs];
     const real lcl = lc[l], lcr = lc[rs], nl = man[l], nr = man[rs];
     const int end = es.seg.mem_end[m];
@@ -530,21 +552,32 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
 // edge order (perm: the cell's entry slots in that order, two bits each, 3 =
 // end), give the factor min(1, max(h, 0) A / sum); the cell then writes the
 // limited transfer sign * |Q dt| * factor of each edge it donates to (every
-// interior edge has exactly one donor) into T[edge].
-template <typename real>
+// interior edge has exactly one donor) into T[edge].  SH: the cell tables (ce*,
+// perm) are shared and indexed by the member-local cell; their edge positions
+// are member-local.
+template <typename real, bool SH>
 __global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
         ForcingTables ft, const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
         const unsigned char* __restrict__ perm, const real* __restrict__ qv,
         real* __restrict__ T, const typename V4<real>::type* __restrict__ state, const real* __restrict__ area_) {
-    const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    const int i = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    int b, m, c0 = 0, eb = 0;
+    if constexpr (SH) {
+        b = shared_block(seg, i, m);
+        c0 = m * seg.sh_items;
+        eb = m * seg.sh_link;
+    } else {
+        b = i;
+        m = seg.blk_member[b];
+    }
     const int c = b * blockDim.x + threadIdx.x;
-    const int m = seg.blk_member[b];
-    const int ent[3] = {ce0[c], ce1[c], ce2[c]};
+    const int cl = c - c0;
+    const int ent[3] = {ce0[cl], ce1[cl], ce2[cl]};
     real qk[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) qk[k] = qv[ent[k] >= 0 ? (ent[k] >> 2) : 0];
+    for (int k = 0; k < 3; ++k) qk[k] = qv[ent[k] >= 0 ? (ent[k] >> 2) + eb : 0];
     const real h = state[c].x, A = area_[c];
-    const unsigned order = perm[c];
+    const unsigned order = perm[cl];
     const int mend = seg.mem_end[m];
     FH_ISSUE_LOADS_OF("s"(ce0), "s"(ce1), "s"(ce2), "s"(perm), "s"(qv), "s"(state), "s"(area_));
     Ctl* ctl = ctls + m;
@@ -577,7 +610,7 @@ __global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<rea
     const real factor = vmin(real(1), (vmax(h, zero) * A) / vmax(sum, p.tiny));
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-        if (donor[k]) T[ent[k] >> 2] = (req[k] >= zero ? real(1) : real(-1)) * (fabs(req[k]) * factor);
+        if (donor[k]) T[(ent[k] >> 2) + eb] = (req[k] >= zero ? real(1) : real(-1)) * (fabs(req[k]) * factor);
 }
 
 // Error-free transformation a + b = s + err (TwoSum, Knuth).
@@ -613,7 +646,8 @@ struct CellSources {
 // LI: the local-inertial surface (triangular_backend.py:1249-1286) in place of
 // the flux divergence and friction: rec holds {Q, Q nx, Q ny} per edge and recR
 // the donor-limited transfer T of each interior edge (k_li_flux, k_li_donor).
-template <typename real, bool COMP, bool CLS, bool LI>
+// SH: the cell->edge table is shared (as in k_li_donor).
+template <typename real, bool COMP, bool CLS, bool LI, bool SH>
 __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
         ForcingTables ft,
         const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
@@ -622,15 +656,23 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
         real* __restrict__ wet, const real* __restrict__ area_, const real* __restrict__ lc_,
         CellSources<real> src, double* __restrict__ part) {
     typedef typename V4<real>::type V;
-    const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    const int i = xcd_block(blockIdx.x, gridDim.x, p.remap);
+    int b, m, c0 = 0, eb = 0;
+    if constexpr (SH) {
+        b = shared_block(seg, i, m);
+        c0 = m * seg.sh_items;
+        eb = m * seg.sh_link;
+    } else {
+        b = i;
+        m = seg.blk_member[b];
+    }
     const int c = b * blockDim.x + threadIdx.x;
-    const int m = seg.blk_member[b];
     const real zero = real(0), eps = p.eps;
     // every load of the cell (the grid covers the padded cell space exactly;
     // padding and inactive cells carry no table entries) ahead of the control block
     V s = state[c];
     const real wt0 = wet[c];
-    const int ent[3] = {ce0[c], ce1[c], ce2[c]};
+    const int ent[3] = {ce0[c - c0], ce1[c - c0], ce2[c - c0]};
     // The three gathers are unconditional (a missing entry reads record 0 and
     // is ignored below), so they issue back to back with no branch between
     // them; recR is gathered for left entries too and ignored there.
@@ -639,7 +681,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const int en = ent[k];
-        const int pos = en >= 0 ? (en >> 2) : 0;
+        const int pos = en >= 0 ? (en >> 2) + eb : 0;
         rr[k] = rec[pos];
         rR[k] = recR[pos];
     }

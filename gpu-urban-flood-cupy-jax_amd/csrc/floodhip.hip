@@ -788,10 +788,22 @@ public:
                 m.cpos.clear(); m.epos.clear(); m.npos.clear(); m.lpos.clear();
             }
         }
+        // A batch whose members all have the same mesh (same device layout) keeps
+        // one copy of the edge tables and the cell->edge table, with
+        // member-local indices; its members keep their given order.
+        shared = P.share_mesh != 0 && M > 1 && same_mesh();
+        if (shared) {
+            // exact block -> member division (shared_block): blocks * blocks per member < 2^40
+            const long long cb = (mem[0].C + bs - 1) / bs, eb = (mem[0].E + (int64_t)bs * ept - 1) / ((int64_t)bs * ept);
+            const long long mx = std::max(cb, eb);
+            if (cb == 0 || eb == 0 || (long double)M * mx * mx >= (long double)(1ull << 40)) shared = false;
+        }
         // member order in the concatenation
         order.resize(M);
         std::iota(order.begin(), order.end(), 0);
-        if (P.pack_order == 1) {
+        if (shared) {
+            // one size: the given order
+        } else if (P.pack_order == 1) {
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return mem[a].C > mem[b].C; });
         } else if (P.pack_order == 2) {   // alternate largest / smallest
             std::vector<int> bysize(order);
@@ -850,9 +862,11 @@ public:
         }
         const hipStream_t s = stream;
         // ---------------- surface arrays
-        std::vector<int> el(edges.total, 0), er(edges.total, fh::kGhostCopy);
-        std::vector<real> nx(edges.total, 0), ny(edges.total, 0), len(edges.total, 0);
-        std::vector<int> ce0(cells.total, fh::kCellInactive), ce1(cells.total, -1), ce2(cells.total, -1);
+        // edge tables and the cell->edge table: one member's worth when shared
+        const int sh_e = shared ? edges.total / M : edges.total, sh_c = shared ? cells.total / M : cells.total;
+        std::vector<int> el(sh_e, 0), er(sh_e, fh::kGhostCopy);
+        std::vector<real> nx(sh_e, 0), ny(sh_e, 0), len(sh_e, 0);
+        std::vector<int> ce0(sh_c, fh::kCellInactive), ce1(sh_c, -1), ce2(sh_c, -1);
         std::vector<V> st(cells.total);
         std::vector<real> wet(cells.total, 0), area(cells.total, 1), n2(cells.total, 0), lc(cells.total, 0),
             rainm(cells.total, 0), f0(cells.total, 0), fc(cells.total, 0), hk(cells.total, 0);
@@ -862,11 +876,14 @@ public:
         for (int i = 0; i < M; ++i) {
             const Member<real>& m = mem[order[i]];
             const int cb = cells.base[i], eb = edges.base[i];
+            // offsets of the tables' cell and edge indices (0: member-local, shared)
+            const int tcb = shared ? 0 : cb, teb = shared ? 0 : eb;
+            const bool fill = !shared || i == 0;
             const bool perm = !m.epos.empty();
-            for (int64_t e = 0; e < m.E; ++e) {
-                const int64_t q = eb + (perm ? m.epos[(size_t)e] : e);
-                el[q] = m.cp(m.el[e]) + cb;
-                er[q] = m.er[e] >= 0 ? m.cp(m.er[e]) + cb : m.er[e];
+            for (int64_t e = 0; fill && e < m.E; ++e) {
+                const int64_t q = teb + (perm ? m.epos[(size_t)e] : e);
+                el[q] = m.cp(m.el[e]) + tcb;
+                er[q] = m.er[e] >= 0 ? m.cp(m.er[e]) + tcb : m.er[e];
                 nx[q] = m.nx[e]; ny[q] = m.ny[e]; len[q] = m.len[e];
             }
             // entry = (edge position << 2) | (boundary edge << 1) | right side
@@ -875,11 +892,14 @@ public:
                 const int staged = x >> 1;
                 const int p = perm ? m.epos[(size_t)staged] : staged;
                 const int ghost = staged >= m.E_int ? 1 : 0;
-                return ((p + eb) << 2) | (ghost << 1) | (x & 1);
+                return ((p + teb) << 2) | (ghost << 1) | (x & 1);
             };
             for (int64_t c = 0; c < m.C; ++c) {
                 const int64_t q = cb + m.cp((int)c);
-                ce0[q] = enc(m.ce0[c]); ce1[q] = enc(m.ce1[c]); ce2[q] = enc(m.ce2[c]);
+                if (fill) {
+                    const int64_t qt = tcb + m.cp((int)c);
+                    ce0[qt] = enc(m.ce0[c]); ce1[qt] = enc(m.ce1[c]); ce2[qt] = enc(m.ce2[c]);
+                }
                 st[q].x = m.h[c]; st[q].y = m.hu[c]; st[q].z = m.hv[c]; st[q].w = m.bed[c];
                 wet[q] = m.wet[c]; area[q] = m.area[c]; n2[q] = m.n2[c]; lc[q] = m.lc[c];
                 rainm[q] = m.rainm[c]; f0[q] = m.f0[c]; fc[q] = m.fc[c]; hk[q] = m.hk[c];
@@ -899,13 +919,14 @@ public:
             // Manning n, and each cell's interior entry slots in ascending edge order
             // (two bits per slot, 3 = end) for the donor sums
             std::vector<real> man(cells.total, 0);
-            std::vector<unsigned char> perm(cells.total, 0xFF);
+            std::vector<unsigned char> perm(sh_c, 0xFF);
             for (int i = 0; i < M; ++i) {
                 const Member<real>& m = mem[order[i]];
                 const int cb = cells.base[i];
                 for (int64_t c = 0; c < m.C; ++c) {
                     const int64_t q = cb + m.cp((int)c);
                     man[q] = m.n1[c];
+                    if (shared && i > 0) continue;
                     const int t[3] = {m.ce0[c], m.ce1[c], m.ce2[c]};
                     int slot[3], key[3], n = 0;
                     for (int k = 0; k < 3; ++k)
@@ -917,7 +938,7 @@ public:
                         }
                     unsigned code = 0xFF;
                     for (int k = 0; k < n; ++k) code = (code & ~(3u << (2 * k))) | ((unsigned)slot[k] << (2 * k));
-                    perm[q] = (unsigned char)code;
+                    perm[(shared ? 0 : cb) + m.cp((int)c)] = (unsigned char)code;
                 }
             }
             d_cell_n.set(man, s);
@@ -968,6 +989,42 @@ public:
         packed = true;
         dirty = false;
         any_drainage = nodes.total > 0;
+    }
+
+    // Every member has member 0's mesh in the same device layout: the tables a
+    // shared-mesh batch stores once (edges, cell->edge entries) would be equal.
+    bool same_mesh() const {
+        const Member<real>& a = mem[0];
+        auto eq = [](const auto& x, const auto& y) {
+            return x.size() == y.size() && (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(x[0])) == 0);
+        };
+        for (int i = 1; i < M; ++i) {
+            const Member<real>& b = mem[(size_t)i];
+            if (b.C != a.C || b.E != a.E || b.E_int != a.E_int) return false;
+            if (!eq(a.el, b.el) || !eq(a.er, b.er) || !eq(a.nx, b.nx) || !eq(a.ny, b.ny) || !eq(a.len, b.len) ||
+                !eq(a.ce0, b.ce0) || !eq(a.ce1, b.ce1) || !eq(a.ce2, b.ce2) || !eq(a.cpos, b.cpos) ||
+                !eq(a.epos, b.epos))
+                return false;
+        }
+        return true;
+    }
+
+    // Cell and edge index spaces as the surface kernels see them (with the
+    // shared-mesh strides when the batch shares one mesh).
+    fh::Seg cell_seg() const {
+        fh::Seg g = seg_cells.seg();
+        if (shared) set_shared(g, cells.mem_nblk[0], cells.total / M, edges.total / M);
+        return g;
+    }
+    fh::Seg edge_seg() const {
+        fh::Seg g = seg_edges.seg();
+        if (shared) set_shared(g, edges.mem_nblk[0], edges.total / M, cells.total / M);
+        return g;
+    }
+    void set_shared(fh::Seg& g, int nblk, int items, int link) const {
+        g.sh_members = M; g.sh_nblk = nblk; g.sh_items = items; g.sh_link = link;
+        g.sh_rot = nblk / M;
+        g.sh_recip = (1ull << 40) / (unsigned long long)nblk + 1;
     }
 
     // Class table of the per-cell source tuples (n^2, m, f0, fc, k), compared bit
@@ -1490,40 +1547,52 @@ public:
     }
     void launch_li_flux() {
         if (edges.blocks == 0) return;
-        fh::EdgeSeg es{seg_edges.seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
-        go(fh::k_li_flux<real>, edges.blocks, bs, ctl, kp, es, par, forcing_tables(), d_el.p, d_er.p, d_nx.p,
-           d_ny.p, d_len.p, d_state.p, d_lc.p, d_cell_n.p, d_rec.p, d_qli.p);
+        fh::EdgeSeg es{edge_seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
+#define FH_LIFLUX(SH_)                                                                                     \
+        go(fh::k_li_flux<real, SH_>, edges.blocks, bs, ctl, kp, es, par, forcing_tables(), d_el.p, d_er.p,   \
+           d_nx.p, d_ny.p, d_len.p, d_state.p, d_lc.p, d_cell_n.p, d_rec.p, d_qli.p)
+        if (shared) FH_LIFLUX(true); else FH_LIFLUX(false);
+#undef FH_LIFLUX
     }
     void launch_li_donor() {
-        go(fh::k_li_donor<real>, cells.blocks, bs, ctl, kp, seg_cells.seg(), par, forcing_tables(), d_ce0.p,
-           d_ce1.p, d_ce2.p, d_perm.p, d_qli.p, d_recR.p, d_state.p, d_area.p);
+#define FH_DONOR(SH_)                                                                                      \
+        go(fh::k_li_donor<real, SH_>, cells.blocks, bs, ctl, kp, cell_seg(), par, forcing_tables(), d_ce0.p, \
+           d_ce1.p, d_ce2.p, d_perm.p, d_qli.p, d_recR.p, d_state.p, d_area.p)
+        if (shared) FH_DONOR(true); else FH_DONOR(false);
+#undef FH_DONOR
     }
     void launch_flux() {
         if (edges.blocks + orphans.blocks == 0) return;
-        fh::EdgeSeg es{seg_edges.seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
+        fh::EdgeSeg es{edge_seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
         const unsigned grid = (unsigned)(edges.blocks + orphans.blocks);
-#define FH_FLUX(EPT_)                                                                                    \
-        go(fh::k_flux<real, EPT_>, grid, bs, ctl, kp, es, par, d_el.p, d_er.p, d_nx.p, d_ny.p, d_len.p, \
+#define FH_FLUX(EPT_, SH_)                                                                                    \
+        go(fh::k_flux<real, EPT_, SH_>, grid, bs, ctl, kp, es, par, d_el.p, d_er.p, d_nx.p, d_ny.p, d_len.p, \
            d_state.p, d_lc.p, d_rec.p, d_recR.p)
-        if (ept == 4) FH_FLUX(4);
-        else if (ept == 2) FH_FLUX(2);
-        else FH_FLUX(1);
+#define FH_FLUX_S(EPT_) if (shared) FH_FLUX(EPT_, true); else FH_FLUX(EPT_, false)
+        if (ept == 4) { FH_FLUX_S(4); }
+        else if (ept == 2) { FH_FLUX_S(2); }
+        else { FH_FLUX_S(1); }
+#undef FH_FLUX_S
 #undef FH_FLUX
     }
     void launch_update() {
         const fh::CellSources<real> src{d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p, d_cls.p, d_cls_tab.p, d_cls_hk.p};
-#define FH_UPDATE(COMP_, CLS_, LI_)                                                                         \
-        go(fh::k_update<real, COMP_, CLS_, LI_>, cells.blocks, bs, ctl, kp, seg_cells.seg(), par,                 \
+        const fh::Seg cs = cell_seg();
+#define FH_UPDATE(COMP_, CLS_, LI_, SH_)                                                                    \
+        go(fh::k_update<real, COMP_, CLS_, LI_, SH_>, cells.blocks, bs, ctl, kp, cs, par,                        \
            forcing_tables(), d_ce0.p, d_ce1.p, d_ce2.p,                                                         \
            d_rec.p, d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, d_lc.p, src, d_part_upd.p)
+#define FH_UPDATE_S(COMP_, CLS_, LI_) \
+        if (shared) { FH_UPDATE(COMP_, CLS_, LI_, true); } else { FH_UPDATE(COMP_, CLS_, LI_, false); }
 #define FH_UPDATE_M(COMP_, CLS_) \
-        if (li) { FH_UPDATE(COMP_, CLS_, true); } else { FH_UPDATE(COMP_, CLS_, false); }
+        if (li) { FH_UPDATE_S(COMP_, CLS_, true); } else { FH_UPDATE_S(COMP_, CLS_, false); }
         if (P.compensated) {
             if (use_classes) { FH_UPDATE_M(true, true); } else { FH_UPDATE_M(true, false); }
         } else {
             if (use_classes) { FH_UPDATE_M(false, true); } else { FH_UPDATE_M(false, false); }
         }
 #undef FH_UPDATE_M
+#undef FH_UPDATE_S
 #undef FH_UPDATE
     }
 
@@ -1905,6 +1974,7 @@ public:
     DevArray<real> d_dw_in, d_dw_tw;  // dry-weather step: node inflow and tailwater override
     int n_inlet_slots = 0;
     bool li = false;                  // local-inertial surface mode
+    bool shared = false;              // shared-mesh batch: edge and cell->edge tables stored once
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
     DevArray<unsigned short> d_cls;   // source class per cell (when use_classes)
     DevArray<V> d_cls_tab;            // class -> {n^2, m, f0, fc}
@@ -2004,6 +2074,7 @@ void fh_default_params(fh_params* p) {
     p->cell_order = 1;
     p->source_classes = 1;
     p->surface_mode = 0;
+    p->share_mesh = 1;
 }
 
 const char* fh_version(void) { return "floodhip 0.2.0 (gfx950)"; }

@@ -27,7 +27,7 @@ class FhParams(C.Structure):
         ("block_size", C.c_int32), ("xcd_remap", C.c_int32), ("device", C.c_int32),
         ("edges_per_thread", C.c_int32), ("pack_order", C.c_int32),
         ("cell_order", C.c_int32), ("source_classes", C.c_int32),
-        ("surface_mode", C.c_int32),
+        ("surface_mode", C.c_int32), ("share_mesh", C.c_int32),
     ]
 
 

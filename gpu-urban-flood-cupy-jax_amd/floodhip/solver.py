@@ -82,6 +82,7 @@ class SolverOptions(NamedTuple):
     cell_order: int = 1
     source_classes: int = 1
     surface_mode: str = "full_swe"   # or "local_inertial" (engine.py:219-256)
+    share_mesh: bool = True          # batches of one mesh store its tables once
     cfl: float = 0.85
     cfl_interval: int = 32
     reuse_factor: float = 0.8
@@ -138,6 +139,7 @@ class HipParitySolver:
         if opts.surface_mode not in SURFACE_MODES:
             raise ValueError(f"surface_mode must be one of {sorted(SURFACE_MODES)}, got {opts.surface_mode!r}")
         p.surface_mode = SURFACE_MODES[opts.surface_mode]
+        p.share_mesh = int(bool(opts.share_mesh))
         self._ctx = self.lib.fh_create_batch(C.byref(p), int(members))
         if not self._ctx:
             msg = self.lib.fh_global_error().decode()

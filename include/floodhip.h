@@ -75,6 +75,9 @@ typedef struct {
     int32_t surface_mode;       /* 0: full shallow-water step (v1_parity.py:161-214); 1: local-inertial step
                                    (triangular_backend.py:1181-1291) with the gravity-wave CFL timestep
                                    (engine.py:220-225).  Sources, clamp, drainage and ledger are shared. */
+    int32_t share_mesh;         /* 1: a batch whose members all have the same mesh stores its edge tables and
+                                   cell->edge table once and runs the members' copies of each tile back to
+                                   back (same results); 0: one copy per member */
 } fh_params;
 
 typedef struct {

@@ -56,3 +56,40 @@ def test_members_keep_their_own_dt():
         ens.advance_to(20.0, 600.0)
         steps = [c["global_step"] for c, _ in ens.member_clocks()]
     assert steps[0] == 20 and steps[-1] > 20   # the deep member takes more (smaller) steps
+
+
+def _shared_members(n=3):
+    # one mesh and network (with pipes, inlets and outfalls), per-member rain and Horton
+    base = small_network_scenario(seed=11, n=40, h_init=0.2)
+    return synth.make_shared_mesh_ensemble(base, n_members=n, rain_range_mm=(60.0, 240.0))
+
+
+@pytest.mark.parametrize("opts", [dict(), dict(compensated=True, edges_per_thread=2),
+                                  dict(edges_per_thread=4, block_size=128, cell_order=0),
+                                  dict(surface_mode="local_inertial"),
+                                  dict(surface_mode="local_inertial", compensated=True, block_size=64)])
+def test_shared_mesh_members_equal_plain_runs_bitwise(opts):
+    """A batch of one mesh stores the edge and cell->edge tables once (member-local
+    indices, the members' copies of a tile run back to back); each member must
+    still equal its plain run bit for bit, and the copy-per-member batch too."""
+    members = _shared_members()
+    ref = [_plain(m, **opts) for m in members]
+    bytes_by_mode = {}
+    for share in (True, False):
+        with HipEnsemble(members, steps_per_launch=16, share_mesh=share, **opts) as ens:
+            ens.advance_to(45.0, 600.0, max_steps=100_000)
+            states = ens.member_states()
+            clocks = ens.member_clocks()
+            bytes_by_mode[share] = ens.device_bytes()
+        for m, (st, (clock, ledger)) in enumerate(zip(states, clocks)):
+            rst, rclock, rledger = ref[m]
+            assert clock["global_step"] == rclock["global_step"], (share, m)
+            assert clock["time_s"] == rclock["time_s"], (share, m)
+            for name in ("h", "hu", "hv", "wet_time_s", "node_volume", "link_flow"):
+                assert np.array_equal(getattr(st, name), getattr(rst, name)), (share, m, name)
+            assert np.allclose(np.asarray(ledger), np.asarray(rledger), rtol=1e-12, atol=1e-12), (share, m)
+    # the shared batch holds one copy of the 20 B per edge and 12 B per cell tables
+    # (the device keeps at most every input edge: 90 % is a safe lower bound)
+    s = members[0].surface
+    saved = (len(members) - 1) * (20 * s.edge_left.size + 12 * s.bed.size)
+    assert bytes_by_mode[False] - bytes_by_mode[True] >= 0.9 * saved, bytes_by_mode

@@ -44,6 +44,36 @@ def test_default_params_are_the_reference_constants():
     assert p.max_speed == 5.0 and p.inlet_orifice_q == 0.62 * 0.04 and p.slot_wave_speed == 30.0
 
 
+def test_param_structs_match_the_header_layout(tmp_path):
+    """The ctypes mirrors of fh_params / fh_clock have the C layout: every field
+    at the offset gcc gives it, and the same total size."""
+    import ctypes
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = []
+    for struct, cls in (("fh_params", _lib.FhParams), ("fh_clock", _lib.FhClock)):
+        lines.append(f'printf("{struct} size %zu\\n", sizeof({struct}));')
+        for name, _ in cls._fields_:
+            lines.append(f'printf("{struct} {name} %zu\\n", offsetof({struct}, {name}));')
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "floodhip.h"\nint main(void) {\n'
+                   + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(w[:2]): int(w[2]) for w in (ln.split() for ln in out) if len(w) == 3}
+    for struct, cls in (("fh_params", _lib.FhParams), ("fh_clock", _lib.FhClock)):
+        assert got[(struct, "size")] == ctypes.sizeof(cls), struct
+        for name, _ in cls._fields_:
+            assert got[(struct, name)] == getattr(cls, name).offset, (struct, name)
+    _ensure_built()
+    p = _lib.FhParams()
+    _lib.load().fh_default_params(ctypes.byref(p))
+    assert p.share_mesh == 1 and p.surface_mode == 0 and p.cell_order == 1
+
+
 def test_fails_closed_without_gpu():
     _ensure_built()
     lib = _lib.load()
