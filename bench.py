@@ -105,9 +105,11 @@ def batch_leg(base, args, plain_rate):
     opts = dict(compensated=True, use_graph=True)
     tuned = None
     if not args.no_autotune:
-        best, table = autotune(members[:4], steps=32, warmup=8, log=log, compensated=True)
+        # tuned on the whole batch: a shared-mesh batch's block order and cache
+        # reuse depend on its member count, so a sample of members mispredicts
+        best, table = autotune(members, steps=32, warmup=8, log=log, compensated=True)
         opts.update(best)
-        tuned = {"best": best, "table": [[o, r] for o, r in table], "sample_members": 4}
+        tuned = {"best": best, "table": [[o, r] for o, r in table], "sample_members": len(members)}
     log(f"batched ensemble: {len(members)} members, options {opts}")
     rate, el, steps, prof = _ensemble_rate(members, args.batch_steps, 20, opts)
     # sequential plain baseline with the same state precision: one member alone, default launch options

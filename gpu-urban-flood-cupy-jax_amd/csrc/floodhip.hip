@@ -780,9 +780,15 @@ public:
 
     void pack() {
         drop_graph();
-        for (auto& m : mem) {
+        for (size_t i = 0; i < mem.size(); ++i) {
+            Member<real>& m = mem[i];
             if (P.cell_order == 1 && m.C > 0) {
-                if (m.bfs.empty()) locality_order(m);
+                if (m.bfs.empty()) {
+                    // a member with an earlier member's mesh takes its order
+                    for (size_t j = 0; j < i && m.bfs.empty(); ++j)
+                        if (!mem[j].bfs.empty() && same_surface(mem[j], m)) m.bfs = mem[j].bfs;
+                    if (m.bfs.empty()) locality_order(m);
+                }
                 assign_positions(m);
             } else {
                 m.cpos.clear(); m.epos.clear(); m.npos.clear(); m.lpos.clear();
@@ -991,19 +997,23 @@ public:
         any_drainage = nodes.total > 0;
     }
 
+    template <class A>
+    static bool same_array(const A& x, const A& y) {
+        return x.size() == y.size() && (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(x[0])) == 0);
+    }
+    // The staged surfaces of a and b are the same mesh: sizes, edge tables and
+    // cell->edge entries equal bit for bit.
+    static bool same_surface(const Member<real>& a, const Member<real>& b) {
+        return b.C == a.C && b.E == a.E && b.E_int == a.E_int && same_array(a.el, b.el) && same_array(a.er, b.er) &&
+               same_array(a.nx, b.nx) && same_array(a.ny, b.ny) && same_array(a.len, b.len) &&
+               same_array(a.ce0, b.ce0) && same_array(a.ce1, b.ce1) && same_array(a.ce2, b.ce2);
+    }
     // Every member has member 0's mesh in the same device layout: the tables a
     // shared-mesh batch stores once (edges, cell->edge entries) would be equal.
     bool same_mesh() const {
-        const Member<real>& a = mem[0];
-        auto eq = [](const auto& x, const auto& y) {
-            return x.size() == y.size() && (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(x[0])) == 0);
-        };
         for (int i = 1; i < M; ++i) {
             const Member<real>& b = mem[(size_t)i];
-            if (b.C != a.C || b.E != a.E || b.E_int != a.E_int) return false;
-            if (!eq(a.el, b.el) || !eq(a.er, b.er) || !eq(a.nx, b.nx) || !eq(a.ny, b.ny) || !eq(a.len, b.len) ||
-                !eq(a.ce0, b.ce0) || !eq(a.ce1, b.ce1) || !eq(a.ce2, b.ce2) || !eq(a.cpos, b.cpos) ||
-                !eq(a.epos, b.epos))
+            if (!same_surface(mem[0], b) || !same_array(mem[0].cpos, b.cpos) || !same_array(mem[0].epos, b.epos))
                 return false;
         }
         return true;
