@@ -52,7 +52,7 @@ def measured_traffic(kernel):
     return None
 
 
-def survey_bytes(kernel, E, C, compensated=False):
+def algorithmic_bytes(kernel, E, C, compensated=False):
     """Algorithmic bytes per launch (float32): each required element once per step."""
     if kernel == "flux":
         return 44.0 * E + 20.0 * C
@@ -121,7 +121,7 @@ def batch_leg(base, args, plain_rate):
     M = len(members)
     kern = {}
     for k in prof:
-        b = survey_bytes(k["kernel"], E * M, C * M, True)
+        b = algorithmic_bytes(k["kernel"], E * M, C * M, True)
         kern[k["kernel"]] = {"avg_us": round(1e3 * k["avg_ms"], 3),
                              "GBps": round(b / (k["avg_ms"] * 1e-3) / 1e9, 1) if b else None}
     return {"workload": f"{M} members x {C} triangles (shared mesh and network, own rain totals 40-200 mm/h "
@@ -244,13 +244,13 @@ def main():
         log(f"kernel {k['kernel']:>18s}: {1e3 * k['avg_ms']:8.2f} us  x{k['launches']}")
     by_time = sorted([k for k in prof if k["kernel"] in ("flux", "update")], key=lambda k: -k["avg_ms"])
     dom = by_time[0]
-    ab = survey_bytes(dom["kernel"], E, C, args.compensated)
+    ab = algorithmic_bytes(dom["kernel"], E, C, args.compensated)
     achieved = ab / (dom["avg_ms"] * 1e-3) / 1e9
     kernels = {k["kernel"]: {"avg_us": round(1e3 * k["avg_ms"], 3),
-                             "algorithmic_bytes": survey_bytes(k["kernel"], E, C, args.compensated),
-                             "GBps": (round(survey_bytes(k["kernel"], E, C, args.compensated)
+                             "algorithmic_bytes": algorithmic_bytes(k["kernel"], E, C, args.compensated),
+                             "GBps": (round(algorithmic_bytes(k["kernel"], E, C, args.compensated)
                                             / (k["avg_ms"] * 1e-3) / 1e9, 1)
-                                      if survey_bytes(k["kernel"], E, C, args.compensated) else None)}
+                                      if algorithmic_bytes(k["kernel"], E, C, args.compensated) else None)}
                for k in prof}
     s.close()
     local_inertial = None if args.no_li else local_inertial_leg(inp, args)

@@ -1,5 +1,5 @@
 """The reference's eight acceptance gates (parity_validation.py:179-192), GPU run
-against the oracle run of the same event, in both surface modes: the SURVEY config-1 mesh and storm
+against the oracle run of the same event, in both surface modes: the plain 2 km square mesh and its storm
 (20,000 triangles, 50 mm/h for 600 s, CFL dt) and a coupled catchment with the
 drainage network (rain, inlets, links, outfalls, overflow; 20 min).
 
