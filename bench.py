@@ -116,6 +116,11 @@ def batch_leg(base, args, plain_rate):
     seq_rate, _, _, _ = _ensemble_rate([members[0]], args.batch_steps, 20, dict(compensated=True, use_graph=True))
     log(f"batched: {rate:.4e} tri-steps/s over {steps} member-steps in {el:.3f}s; sequential plain "
         f"{seq_rate:.4e} ({rate / seq_rate:.2f}x)")
+    # accuracy of the same batch against its FP64 run: 120 fixed 0.5 s steps
+    from floodhip.precision import ensemble_vs_fp64
+    fp64 = ensemble_vs_fp64(members, steps=120, dt=0.5, use_graph=True)
+    log(f"batched vs FP64 (120 x 0.5 s): worst stored-volume rel {fp64['worst']['stored_volume_rel']:.3e}, "
+        f"surface-volume rel {fp64['worst']['surface_volume_rel']:.3e}, depth Linf {fp64['worst']['depth_linf_m']:.3e} m")
     C = int(base.surface.cell_area.shape[0])
     E = int(base.surface.edge_left.shape[0])
     M = len(members)
@@ -128,7 +133,8 @@ def batch_leg(base, args, plain_rate):
                         f"and Horton factors), float32 compensated state, FP64 ledgers, hipGraph",
             "members": M, "value": rate, "unit": "triangle-steps/s", "timed_member_steps": steps,
             "seconds": el, "sequential_plain_value": seq_rate, "speedup_vs_sequential_plain": rate / seq_rate,
-            "options": opts, "autotune": tuned, "kernels": kern}
+            "options": opts, "autotune": tuned, "kernels": kern,
+            "vs_fp64": {"fixed_dt_s": fp64["fixed_dt_s"], "steps": fp64["steps"], "worst_member": fp64["worst"]}}
 
 
 def local_inertial_leg(inp, args):

@@ -93,3 +93,15 @@ def test_shared_mesh_members_equal_plain_runs_bitwise(opts):
     s = members[0].surface
     saved = (len(members) - 1) * (20 * s.edge_left.size + 12 * s.bed.size)
     assert bytes_by_mode[False] - bytes_by_mode[True] >= 0.9 * saved, bytes_by_mode
+
+
+def test_shared_mesh_batch_volume_against_fp64():
+    """The compensated float32 batch against the same batch in FP64, same fixed dt:
+    every member's stored volume (surface + nodes) within 1e-5 relative."""
+    from floodhip.precision import ensemble_vs_fp64
+    res = ensemble_vs_fp64(_shared_members(4), steps=200, dt=0.5)
+    worst = res["worst"]
+    assert worst["stored_volume_rel"] <= 1e-5, res
+    assert worst["surface_volume_rel"] <= 1e-5, res
+    assert worst["depth_linf_m"] <= 1e-3, res
+    assert all(m["steps"] == 200 for m in res["members"])
