@@ -259,6 +259,11 @@ def main():
                                       if algorithmic_bytes(k["kernel"], E, C, args.compensated) else None)}
                for k in prof}
     s.close()
+    # the plain run's accuracy against its FP64 run (120 fixed 0.5 s steps)
+    from floodhip.precision import ensemble_vs_fp64
+    fp64 = ensemble_vs_fp64([inp], steps=120, dt=0.5, compensated=args.compensated, use_graph=True)
+    log(f"plain vs FP64 (120 x 0.5 s): stored-volume rel {fp64['worst']['stored_volume_rel']:.3e}, "
+        f"depth Linf {fp64['worst']['depth_linf_m']:.3e} m")
     local_inertial = None if args.no_li else local_inertial_leg(inp, args)
     batched = None
     if not args.no_batch and args.batch_members > 0:
@@ -306,6 +311,7 @@ def main():
                      "traffic": None if args.compensated else measured_traffic(dom["kernel"])},
         "kernels": kernels,
         "simulated_s_per_wall_s": (c1["time_s"] - c0["time_s"]) / el,
+        "vs_fp64": {"fixed_dt_s": fp64["fixed_dt_s"], "steps": fp64["steps"], **fp64["members"][0]},
         "local_inertial": local_inertial,
         "copy_probe": copy,
         "batched_shared_mesh": batched,

@@ -88,9 +88,10 @@ def study(duration=120.0, inp=None) -> dict:
                         "flat frictionless bed, reflective walls", "duration_s": duration, "forms": forms}
 
 
-def ensemble_vs_fp64(members, steps: int = 120, dt: float = 0.5, **opts) -> dict:
-    """A batch in float32 with the compensated update against the same batch in
-    FP64 (the same kernels for double), both with the same fixed dt, so every
+def ensemble_vs_fp64(members, steps: int = 120, dt: float = 0.5, compensated: bool = True, **opts) -> dict:
+    """A batch in float32 (with the compensated update unless compensated=False)
+    against the same batch in FP64 (the same kernels for double), both with the
+    same fixed dt, so every
     member sits at the same simulated time.  Per member: the relative error of
     the stored volume (surface sum h * A plus the node volumes) and of the
     surface volume alone, and the depth L2 / Linf errors.  Returns the worst
@@ -98,7 +99,8 @@ def ensemble_vs_fp64(members, steps: int = 120, dt: float = 0.5, **opts) -> dict
     from .solver import HipEnsemble
     runs = {}
     for prec in (32, 64):
-        with HipEnsemble(members, precision=prec, compensated=prec == 32, fixed_dt=dt, **opts) as ens:
+        with HipEnsemble(members, precision=prec, compensated=compensated and prec == 32, fixed_dt=dt,
+                         **opts) as ens:
             ens.advance_to(1.0e9, 1.0e9, max_steps=steps, raise_on_budget=False)
             out = []
             for m in range(ens.members):

@@ -12,9 +12,10 @@ Each .npz holds every field of the input NamedTuples (``surface.*``,
 ``coupling.*``, ``pipe.*``, ``recharge.*``, ``state.*``, ``forcing.*``), the step
 schedule (``dt``, ``rain``, ``q``: one row per step) and the expected outputs
 (``out.<state field>`` and ``ledger.<field>``, the sum of the per-step ledger
-terms in float64).
+terms in float64), and ``surface_mode`` when the case runs the local-inertial
+surface step.
 
-    python tests/golden/make_golden.py          # rewrite the fixtures
+    python tests/golden/make_golden.py [NAME ...]   # rewrite the fixtures (all, or the named ones)
 """
 
 from __future__ import annotations
@@ -58,12 +59,13 @@ def schedule(inp, steps, dt_of, external=True):
     return np.asarray(dts, np.float64), np.asarray(rains, np.float64), np.stack(qs).reshape(steps, nrx)
 
 
-def run_oracle(inp, dts, rains, qs, dtype):
+def run_oracle(inp, dts, rains, qs, dtype, surface_mode="full_swe"):
     io = to_oracle(inp, dtype)
     st = io[4]
     ledger = orc.zero_ledger()
     for dt, rain, q in zip(dts, rains, qs):
-        st, diag = oracle_step(io, st, float(dt), rain=float(rain), q=np.asarray(q, dtype), dtype=dtype)
+        st, diag = oracle_step(io, st, float(dt), rain=float(rain), q=np.asarray(q, dtype), dtype=dtype,
+                               surface_mode=surface_mode)
         for k in orc.LEDGER_FIELDS:
             ledger[k] += float(diag[k])
     return st, ledger
@@ -77,6 +79,14 @@ CASES = {
     # float64 kernels against the FP64 oracle, fixed dt
     "coupled_f64": dict(make=lambda: small_network_scenario(seed=21, h_init=0.2, with_external=False),
                         steps=100, dt_of=lambda i: 0.4, dtype=np.float64, external=False),
+    # the local-inertial surface step (triangular_backend.py:1181-1291) with the
+    # same sources and drainage, float32 and float64
+    "coupled_li_f32": dict(make=lambda: small_network_scenario(seed=13, h_init=0.2), steps=25,
+                           dt_of=lambda i: 0.3 if i % 2 else 0.45, dtype=np.float32, external=True,
+                           surface_mode="local_inertial"),
+    "coupled_li_f64": dict(make=lambda: small_network_scenario(seed=29, h_init=0.2, with_external=False),
+                           steps=60, dt_of=lambda i: 0.4, dtype=np.float64, external=False,
+                           surface_mode="local_inertial"),
 }
 
 
@@ -84,16 +94,19 @@ def build(name):
     case = CASES[name]
     inp = case["make"]()
     dts, rains, qs = schedule(inp, case["steps"], case["dt_of"], case["external"])
-    st, ledger = run_oracle(inp, dts, rains, qs, case["dtype"])
+    mode = case.get("surface_mode", "full_swe")
+    st, ledger = run_oracle(inp, dts, rains, qs, case["dtype"], mode)
     arrays = flatten_inputs(inp)
     arrays.update({"dt": dts, "rain": rains, "q": qs, "dtype": np.asarray(np.dtype(case["dtype"]).name)})
+    if mode != "full_swe":
+        arrays["surface_mode"] = np.asarray(mode)
     arrays.update({f"out.{k}": np.asarray(v) for k, v in st._asdict().items()})
     arrays.update({f"ledger.{k}": np.float64(v) for k, v in ledger.items()})
     return arrays
 
 
-def main():
-    for name in CASES:
+def main(names=None):
+    for name in names or CASES:
         arrays = build(name)
         path = HERE / f"{name}.npz"
         np.savez_compressed(path, **arrays)
@@ -101,4 +114,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -179,7 +179,8 @@ _GROUP_TYPES = {"surface": types.SurfaceTopology, "coupling": types.CouplingTopo
 
 def load_golden(name):
     """A fixture written by tests/golden/make_golden.py: (LoadedParityInputs, schedule
-    dict with dt/rain/q rows, expected ParityState, expected ledger dict, dtype)."""
+    dict with dt/rain/q rows and the surface mode, expected ParityState, expected
+    ledger dict, dtype)."""
     with np.load(GOLDEN_DIR / f"{name}.npz", allow_pickle=False) as z:
         a = {k: z[k] for k in z.files}
     groups = {}
@@ -190,7 +191,9 @@ def load_golden(name):
                                    groups["initial_state"], groups["forcing"], {}, {})
     expected = types.ParityState(*[a[f"out.{f}"] for f in types.ParityState._fields])
     ledger = {k: float(a[f"ledger.{k}"]) for k in orc.LEDGER_FIELDS}
-    return inp, {"dt": a["dt"], "rain": a["rain"], "q": a["q"]}, expected, ledger, np.dtype(str(a["dtype"]))
+    mode = str(a["surface_mode"]) if "surface_mode" in a else "full_swe"
+    sched = {"dt": a["dt"], "rain": a["rain"], "q": a["q"], "surface_mode": mode}
+    return inp, sched, expected, ledger, np.dtype(str(a["dtype"]))
 
 
 def strip_network(inp, links=True, inlets=True):

@@ -5,7 +5,8 @@ ledger after a stored step schedule.  Re-running the oracle on the stored inputs
 must reproduce them: float64 to 1e-12 m (and 1e-12 relative in the ledger),
 float32 to 2e-6 m, so a change to the oracle's arithmetic shows up here.  The
 stored inputs are also checked to be what the scenario generator builds, so a
-silent change to the generator is caught too."""
+silent change to the generator is caught too.  The coupled_li_* fixtures run the
+local-inertial surface step."""
 
 import numpy as np
 import pytest
@@ -13,7 +14,7 @@ import pytest
 from helpers import load_golden, oracle_step, small_network_scenario, to_oracle
 from oracle import v1_parity_np as orc
 
-CASES = ["coupled_f32", "coupled_f64"]
+CASES = ["coupled_f32", "coupled_f64", "coupled_li_f32", "coupled_li_f64"]
 
 
 def _rerun(inp, sched, dtype):
@@ -21,7 +22,8 @@ def _rerun(inp, sched, dtype):
     st = io[4]
     ledger = orc.zero_ledger()
     for dt, rain, q in zip(sched["dt"], sched["rain"], sched["q"]):
-        st, diag = oracle_step(io, st, float(dt), rain=float(rain), q=np.asarray(q, dtype), dtype=dtype)
+        st, diag = oracle_step(io, st, float(dt), rain=float(rain), q=np.asarray(q, dtype), dtype=dtype,
+                               surface_mode=sched["surface_mode"])
         for k in orc.LEDGER_FIELDS:
             ledger[k] += float(diag[k])
     return st, ledger
@@ -47,6 +49,8 @@ def test_oracle_reproduces_fixture(name):
 @pytest.mark.parametrize("name,make", [
     ("coupled_f32", lambda: small_network_scenario(seed=7)),
     ("coupled_f64", lambda: small_network_scenario(seed=21, h_init=0.2, with_external=False)),
+    ("coupled_li_f32", lambda: small_network_scenario(seed=13, h_init=0.2)),
+    ("coupled_li_f64", lambda: small_network_scenario(seed=29, h_init=0.2, with_external=False)),
 ])
 def test_fixture_inputs_match_generator(name, make):
     inp, *_ = load_golden(name)

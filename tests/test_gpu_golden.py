@@ -6,7 +6,8 @@ Tolerances: float32 |dh| <= 1e-5 m after 25 steps, node volume 1e-4 relative
 (of max(V, 1 m^3)), link flow 1e-4 relative (of max(|Q|, 1e-3 m^3/s)), ledger
 1e-4 relative; float64 kernels |dh|, |dhu| <= 1e-10 after 100 steps (node
 volumes and link flows cross the boundary as float32: 1e-5 relative), ledger
-1e-6 relative."""
+1e-6 relative.  The same bounds hold for the local-inertial fixtures
+(coupled_li_*, 25 float32 steps / 60 float64 steps)."""
 
 import numpy as np
 import pytest
@@ -18,13 +19,13 @@ from oracle import v1_parity_np as orc
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["coupled_f32", "coupled_f64"])
+@pytest.mark.parametrize("name", ["coupled_f32", "coupled_f64", "coupled_li_f32", "coupled_li_f64"])
 def test_hip_matches_golden(name):
     inp, sched, exp, ledger, dtype = load_golden(name)
     f64 = dtype == np.float64
     got_ledger = dict.fromkeys(orc.LEDGER_FIELDS, 0.0)
     with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
-                         precision=64 if f64 else 32) as s:
+                         precision=64 if f64 else 32, surface_mode=sched["surface_mode"]) as s:
         for dt, rain, q in zip(sched["dt"], sched["rain"], sched["q"]):
             d = s.step(float(dt), float(rain), q if q.size else None)
             for k in orc.LEDGER_FIELDS:
