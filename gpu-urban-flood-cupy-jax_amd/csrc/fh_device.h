@@ -66,7 +66,6 @@ struct Clock {
 // written only by k_fold.
 struct Ctl {
     Clock clk[2];
-    unsigned long long cfl_bits[2];   // running CFL minimum of the step of that parity
     int64_t li_cfl_step[2];           // step whose local-inertial CFL pass filled that slot (-1: none)
     double target;         // stop when t >= target
     double final_t;        // event end time (dt cap final_t - t)
@@ -124,6 +123,18 @@ __device__ __forceinline__ int shared_block(const Seg& s, int i, int& m) {
     return m * s.sh_nblk + t;
 }
 
+// CFL minimum of a step, spread over kCflSlots words of one 128-byte line each,
+// [member][parity][slot]: every block adds its block minimum with one atomic min
+// into slot (block index mod kCflSlots), so about 1/kCflSlots of the blocks meet
+// on each address instead of all of them on one (device-scope atomics on one
+// address serialize); readers take the minimum of the slots.  Idle slots hold
+// +inf: a step's committer resets the next step's slots.
+constexpr int kCflSlots = 64;
+constexpr int kCflStride = 16;     // words per slot: one 128-byte line
+__device__ __forceinline__ unsigned long long* cfl_slot(unsigned long long* base, int m, int parity, int k) {
+    return base + ((size_t)(m * 2 + parity) * kCflSlots + k) * kCflStride;
+}
+
 template <typename real>
 struct KP {                 // kernel constants, pre-rounded to the state precision
     real g, eps, half_g, two_g, cfl, max_speed, tiny, capture_q, pow43, reuse, slot_coef;
@@ -131,6 +142,7 @@ struct KP {                 // kernel constants, pre-rounded to the state precis
     double g_d;              // gravity in double (products of two host scalars, g * dt)
     int interval;
     int remap;
+    unsigned long long* cfl_slots;   // the CFL minimum slots (cfl_slot)
 };
 
 // NaN-propagating max/min, like numpy.maximum / jax.numpy.maximum (fmax/fmin

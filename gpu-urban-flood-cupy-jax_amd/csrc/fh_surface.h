@@ -148,11 +148,30 @@ __device__ __forceinline__ real cell_time(const KP<real>& p, const typename V4<r
 
 // Block minimum, then one atomic min per block on the bits of a non-negative
 // double (which order like the values).
-__device__ __forceinline__ void commit_cfl_min(unsigned long long* bits, double v, double* lds) {
+// Block minimum of v into the CFL slot of block `blk` (member m, step parity).
+__device__ __forceinline__ void commit_cfl_min(unsigned long long* cfl, int m, int parity, int blk, double v,
+                                               double* lds) {
     v = block_min(v, lds);
     if (threadIdx.x == 0)
-        __hip_atomic_fetch_min(bits, (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_min(cfl_slot(cfl, m, parity, blk % kCflSlots), (unsigned long long)__double_as_longlong(v),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The step's CFL minimum (member m, parity): each lane of the wave reads one of
+// the kCflSlots (= 64, the wave width) slots, a butterfly takes their minimum,
+// and the result is made wave-uniform.  The values are non-negative or +inf.
+__device__ __forceinline__ double cfl_slots_min(unsigned long long* cfl, int m, int parity) {
+    static_assert(kCflSlots == 64, "one slot per lane of a 64-wide wave");
+    double v = __longlong_as_double((long long)*cfl_slot(cfl, m, parity, (int)(threadIdx.x & 63)));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double w = __shfl_xor(v, o);
+        v = w < v ? w : v;
+    }
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
 // Rain rate at time t (parity_runner.py:96-102): rate[lo] when lo =
@@ -282,7 +301,8 @@ __device__ __forceinline__ void commit_plan(Ctl* ctl, int parity, const Clock& c
         n.active = (n.t < ctl->target) && (n.interval_step < ctl->max_steps);
     }
     ctl->clk[1 - parity] = n;
-    ctl->cfl_bits[1 - parity] = (unsigned long long)__double_as_longlong(INFINITY);
+    for (int k = 0; k < kCflSlots; ++k)
+        *cfl_slot(p.cfl_slots, m, 1 - parity, k) = (unsigned long long)__double_as_longlong(INFINITY);
     ctl->li_cfl_step[1 - parity] = -1;
 }
 
@@ -311,16 +331,20 @@ __global__ void __launch_bounds__(256) k_cfl(Ctl* __restrict__ ctls, KP<real> p,
         const real local = cell_time<real, LI>(p, state[c], lc[c]);
         if (local < mn) mn = local;
     }
-    // the next step's slot (parity) holds +inf between steps and is left so
-    commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
+    // the next step's slots (parity) hold +inf between steps and are left so
+    commit_cfl_min(p.cfl_slots, m, parity, b - blk0, (double)mn, lds);
     if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
     if (threadIdx.x != 0) return;
     __hip_atomic_store(&ctl->counters[C_BEGIN], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long bits = __hip_atomic_load(&ctl->cfl_bits[parity], __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long bits = ~0ull;
+    for (int k = 0; k < kCflSlots; ++k) {
+        unsigned long long* sl = cfl_slot(p.cfl_slots, m, parity, k);
+        const unsigned long long x = __hip_atomic_load(sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bits = x < bits ? x : bits;
+        __hip_atomic_store(sl, (unsigned long long)__double_as_longlong(INFINITY), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
     ctl->cfl_min = __longlong_as_double((long long)bits);
-    __hip_atomic_store(&ctl->cfl_bits[parity], (unsigned long long)__double_as_longlong(INFINITY), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Edge index space: Seg, plus the blocks after the edge blocks that cover each
@@ -363,7 +387,7 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
             const int c = es.ocells[o];
             mn = cell_cfl(p, state[c], lc[c]);
         }
-        commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
+        commit_cfl_min(p.cfl_slots, m, parity, ob, (double)mn, lds);
         return;
     }
     const int i = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
@@ -433,7 +457,7 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
             }
         }
     }
-    if (eval) commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
+    if (eval) commit_cfl_min(p.cfl_slots, m, parity, b, (double)mn, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -463,7 +487,7 @@ __global__ void __launch_bounds__(256) k_li_cfl(Ctl* __restrict__ ctls, KP<real>
         const real local = cell_cfl_li(p, state[c], lc[c]);
         if (local < mn) mn = local;
     }
-    commit_cfl_min(&ctl->cfl_bits[parity], (double)mn, lds);
+    commit_cfl_min(p.cfl_slots, m, parity, b - blk0, (double)mn, lds);
     if (b == blk0 && threadIdx.x == 0) ctl->li_cfl_step[parity] = ck.step;
 }
 
@@ -504,7 +528,7 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
     Ctl* ctl = ctls + m;
     const Clock ck = ctl->clk[parity];
     const bool eval = ck.active && ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);
-    const double gmin = eval ? __longlong_as_double((long long)ctl->cfl_bits[parity]) : INFINITY;
+    const double gmin = eval ? cfl_slots_min(p.cfl_slots, m, parity) : INFINITY;
     const StepPlan<real> pl = plan_step<real>(ctl, ck, p, ft, m, gmin);
     if (!pl.active) return;
     // an evaluating step whose CFL pass did not run (a graph captured for another
@@ -583,7 +607,7 @@ __global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<rea
     Ctl* ctl = ctls + m;
     const Clock ck = ctl->clk[parity];
     const bool eval = ck.active && ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);
-    const double gmin = eval ? __longlong_as_double((long long)ctl->cfl_bits[parity]) : INFINITY;
+    const double gmin = eval ? cfl_slots_min(p.cfl_slots, m, parity) : INFINITY;
     const StepPlan<real> pl = plan_step<real>(ctl, ck, p, ft, m, gmin);
     if (!pl.active || c >= mend || ent[0] == kCellInactive) return;
     const real zero = real(0), dt = (real)pl.dt;
@@ -710,7 +734,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
     Ctl* ctl = ctls + m;
     const Clock ck = ctl->clk[parity];
     const bool eval = ck.active && ck.eval && !ctl->override_dt && !(p.fixed_dt > 0.0);
-    const double gmin = eval ? __longlong_as_double((long long)ctl->cfl_bits[parity]) : INFINITY;
+    const double gmin = eval ? cfl_slots_min(p.cfl_slots, m, parity) : INFINITY;
     const StepPlan<real> pl = plan_step<real>(ctl, ck, p, ft, m, gmin);
     const bool committer = b == blk0 && threadIdx.x == 0;
     if (!pl.active) {

@@ -361,12 +361,19 @@ public:
                 ck.cached_cfl = 1.0;
                 ck.eval = P.fixed_dt > 0.0 ? 0 : 1;
                 ck.rain.next_end = -INFINITY;
-                std::memcpy(&m.ctl.cfl_bits[q], &inf, sizeof(inf));
                 m.ctl.li_cfl_step[q] = -1;
             }
         }
         for (int m = 0; m < M; ++m) ctl_h[m] = mem[m].ctl;
         FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl) * M, hipMemcpyHostToDevice, stream));
+        // the CFL minimum slots, all +inf
+        {
+            const double inf = INFINITY;
+            unsigned long long ib;
+            std::memcpy(&ib, &inf, sizeof(ib));
+            cfl_inf.assign((size_t)M * 2 * fh::kCflSlots * fh::kCflStride, ib);
+            d_cfl.set(cfl_inf, stream);
+        }
         bs = std::max(64, std::min(256, P.block_size > 0 ? P.block_size : 256));
         bs = (bs / 64) * 64;
         ept = P.edges_per_thread == 2 ? 2 : (P.edges_per_thread == 4 ? 4 : 1);
@@ -390,6 +397,7 @@ public:
         li = P.surface_mode == 1;
         if (li) ept = 1;
         kp.remap = P.xcd_remap;
+        kp.cfl_slots = d_cfl.p;
         FH_CHECK(hipStreamSynchronize(stream));
     }
 
@@ -709,6 +717,9 @@ public:
     }
     void push_ctl() {
         FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl) * M, hipMemcpyHostToDevice, stream));
+        // no step is in flight: every CFL minimum slot starts the next step at +inf
+        FH_CHECK(hipMemcpyAsync(d_cfl.p, cfl_inf.data(), cfl_inf.size() * sizeof(unsigned long long),
+                                hipMemcpyHostToDevice, stream));
         FH_CHECK(hipStreamSynchronize(stream));
     }
     // ctl_h is indexed by member id (not pack position)
@@ -1985,6 +1996,8 @@ public:
     int n_inlet_slots = 0;
     bool li = false;                  // local-inertial surface mode
     bool shared = false;              // shared-mesh batch: edge and cell->edge tables stored once
+    DevArray<unsigned long long> d_cfl;   // CFL minimum slots [member][parity][slot] (fh::cfl_slot)
+    std::vector<unsigned long long> cfl_inf;
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
     DevArray<unsigned short> d_cls;   // source class per cell (when use_classes)
     DevArray<V> d_cls_tab;            // class -> {n^2, m, f0, fc}
