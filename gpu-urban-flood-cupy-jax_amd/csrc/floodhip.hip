@@ -810,10 +810,12 @@ public:
         // member-local indices; its members keep their given order.
         shared = P.share_mesh != 0 && M > 1 && same_mesh();
         if (shared) {
-            // exact block -> member division (shared_block): blocks * blocks per member < 2^40
+            // exact block -> member division (shared_block): blocks * blocks per member < 2^40,
+            // and block index * reciprocal (about M * 2^40) within 64 bits
             const long long cb = (mem[0].C + bs - 1) / bs, eb = (mem[0].E + (int64_t)bs * ept - 1) / ((int64_t)bs * ept);
             const long long mx = std::max(cb, eb);
-            if (cb == 0 || eb == 0 || (long double)M * mx * mx >= (long double)(1ull << 40)) shared = false;
+            if (cb == 0 || eb == 0 || M >= (1 << 20) || (long double)M * mx * mx >= (long double)(1ull << 40))
+                shared = false;
         }
         // member order in the concatenation
         order.resize(M);

@@ -105,3 +105,25 @@ def test_shared_mesh_batch_volume_against_fp64():
     assert worst["surface_volume_rel"] <= 1e-5, res
     assert worst["depth_linf_m"] <= 1e-3, res
     assert all(m["steps"] == 200 for m in res["members"])
+
+
+def test_same_geometry_other_active_cells_is_not_shared():
+    """Members with the same triangles but another active-cell mask have other
+    cell->edge tables: the batch keeps a copy per member (same footprint as with
+    sharing off) and every member still equals its plain run."""
+    a, b = _shared_members(2)
+    act = np.asarray(b.surface.active).copy()
+    act[::7] = False
+    b = b._replace(surface=b.surface._replace(active=act))
+    members = [a, b]
+    ref = [_plain(m) for m in members]
+    footprint = {}
+    for share in (True, False):
+        with HipEnsemble(members, steps_per_launch=16, share_mesh=share) as ens:
+            ens.advance_to(45.0, 600.0, max_steps=100_000)
+            states = ens.member_states()
+            footprint[share] = ens.device_bytes()
+        for m, st in enumerate(states):
+            for name in ("h", "hu", "hv", "node_volume"):
+                assert np.array_equal(getattr(st, name), getattr(ref[m][0], name)), (share, m, name)
+    assert footprint[True] == footprint[False], footprint
