@@ -1911,6 +1911,10 @@ public:
     std::vector<KernelTiming> profile(int steps) override {
         ensure_packed();
         std::vector<KernelTiming> out;
+        // every launch carries start/stop events in its dispatch packet; the launches
+        // run back to back as in a step, and the events are read after the last one
+        struct Pending { std::string name; double bytes; hipEvent_t a, b; };
+        std::vector<Pending> pending;
         auto timed = [&](const char* name, double bytes, auto&& fn) {
             hipEvent_t a, b;
             FH_CHECK(hipEventCreate(&a));
@@ -1920,18 +1924,7 @@ public:
             fn();
             ev_start = ev_stop = nullptr;
             FH_CHECK(hipGetLastError());
-            FH_CHECK(hipEventSynchronize(b));
-            float ms = 0.f;
-            FH_CHECK(hipEventElapsedTime(&ms, a, b));
-            (void)hipEventDestroy(a);
-            (void)hipEventDestroy(b);
-            auto it = std::find_if(out.begin(), out.end(), [&](const KernelTiming& k) { return k.name == name; });
-            if (it == out.end()) {
-                out.push_back(KernelTiming{name, 0.0, bytes, 0});
-                it = out.end() - 1;
-            }
-            it->total_ms += ms;
-            it->launches += 1;
+            pending.push_back(Pending{name, bytes, a, b});
         };
         double Ct = 0, Et = 0;
         for (const auto& m : mem) { Ct += (double)m.C; Et += (double)m.E; }
@@ -1956,6 +1949,19 @@ public:
         fold();
         finalize_pending();
         FH_CHECK(hipStreamSynchronize(stream));
+        for (const Pending& q : pending) {
+            float ms = 0.f;
+            FH_CHECK(hipEventElapsedTime(&ms, q.a, q.b));
+            (void)hipEventDestroy(q.a);
+            (void)hipEventDestroy(q.b);
+            auto it = std::find_if(out.begin(), out.end(), [&](const KernelTiming& k) { return k.name == q.name; });
+            if (it == out.end()) {
+                out.push_back(KernelTiming{q.name, 0.0, q.bytes, 0});
+                it = out.end() - 1;
+            }
+            it->total_ms += ms;
+            it->launches += 1;
+        }
         return out;
     }
 
