@@ -862,10 +862,11 @@ public:
         icl.build(counts_of(icells), 256);
         rcl.build(counts_of(rcells), 256);
         xcl.build(counts_of(xcells), 256);
-        // k_cfl blocks: proportional to the member's cells
+        // k_cfl / k_li_cfl blocks: one per 2,048 of the member's cells (at most 512),
+        // each thread looping over 8 cells; the block minimums spread over the CFL slots
         {
             std::vector<int64_t> nb(M);
-            for (int i = 0; i < M; ++i) nb[i] = std::max<int64_t>(1, std::min<int64_t>(128, (mem[order[i]].C + 8191) / 8192));
+            for (int i = 0; i < M; ++i) nb[i] = std::max<int64_t>(1, std::min<int64_t>(512, (mem[order[i]].C + 2047) / 2048));
             begin_l.base.assign(M, 0); begin_l.count.assign(M, 0);
             begin_l.blk_member.clear(); begin_l.mem_blk0.assign(M, 0); begin_l.mem_nblk.assign(M, 0);
             begin_l.mem_end.assign(M, 0);
