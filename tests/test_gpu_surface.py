@@ -155,3 +155,25 @@ def test_mode_and_dry_weather_argument_errors():
     with HipEnsemble([inp, inp]) as ens:
         with pytest.raises(ValueError):
             ens.dry_weather_step(1.0)
+
+
+@pytest.mark.parametrize("mode", ["full_swe", "local_inertial"])
+def test_cfl_minimum_does_not_leak_between_calls(mode):
+    """The step's CFL minimum lives in per-block slots.  A stable_timestep() call on a
+    fast, deep state followed by a calm state must leave the advance exactly as a
+    solver that only ever held the calm state: same steps, dt, time and depths."""
+    calm = _wet_bump_small()
+    deep = calm.initial_state._replace(h=(calm.initial_state.h + np.float32(8.0)).astype(np.float32),
+                                       hu=np.full_like(calm.initial_state.hu, 3.0))
+    with _solver(calm, surface_mode=mode) as ref, _solver(calm, surface_mode=mode) as s:
+        ref_dt0 = ref.stable_timestep()
+        s.set_state(deep)
+        fast_dt = s.stable_timestep()
+        assert fast_dt < 0.5 * ref_dt0          # the deep state's minimum is much smaller
+        s.set_state(calm.initial_state)
+        for target in (20.0, 45.0):
+            ref.advance_to(target, 600.0)
+            s.advance_to(target, 600.0)
+            assert s.clock() == ref.clock()
+            assert np.array_equal(s.get_state().h, ref.get_state().h)
+        assert s.stable_timestep() == ref.stable_timestep()
