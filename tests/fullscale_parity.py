@@ -6,6 +6,13 @@ Not collected by pytest: the oracle half takes minutes of CPU time.
     python tests/fullscale_parity.py gpu    full_swe OUT.npz [T_END]   # GPU: same run, prints the comparison
     python tests/fullscale_parity.py oracle-steps OUT.npz              # CPU: 20 fixed-dt steps, node volumes per step
     python tests/fullscale_parity.py gpu-steps OUT.npz                 # GPU: per-step node-volume comparison
+    python tests/fullscale_parity.py oracle-gates full_swe OUT.npz     # CPU: the 1-hour event, snapshots
+    python tests/fullscale_parity.py gpu-gates full_swe OUT.npz        # GPU: same event, the eight gates
+
+The gate run starts from a uniform 10 cm layer: the reference's flood-extent CSI is
+intersection / max(union, 1), which is 0 on a snapshot where neither run has a cell
+above 5 cm, so from a dry start the first (t = 1e-6 s) snapshot fails that gate for
+any pair of runs.
 
 The comparison reports step counts, the largest depth difference, the depth RMSE
 over active cells, surface and node volume differences and the ledger terms.
@@ -91,6 +98,39 @@ def run_gpu_steps(out, steps=20, dt=0.5):
         print(json.dumps({"max_abs_dh_m": float(np.max(np.abs(got.h - ref["h"])))}), flush=True)
 
 
+def _gate_inputs():
+    inp = synth.make_production_scale()
+    h0 = np.full(inp.surface.cell_area.shape, 0.1, np.float32)
+    return inp._replace(initial_state=inp.initial_state._replace(h=h0))
+
+
+def run_oracle_gates(mode, out, duration=3600.0):
+    from helpers import oracle_run
+    t0 = time.time()
+    ref = oracle_run(_gate_inputs(), duration, surface_mode=mode)
+    print(f"oracle {mode}: {ref['report']['step_count']} steps in {time.time() - t0:.0f}s", flush=True)
+    np.savez_compressed(out, report=np.asarray(json.dumps(ref["report"])),
+                        **{k: v for k, v in ref.items() if k != "report"})
+
+
+def run_gpu_gates(mode, out, duration=3600.0):
+    from floodhip import gates
+    from floodhip.runner import run_hip_v1_parity
+    inp = _gate_inputs()
+    with np.load(out, allow_pickle=False) as z:
+        ref = {k: z[k] for k in z.files if k != "report"}
+        ref["report"] = json.loads(str(z["report"]))
+    t0 = time.time()
+    cand = run_hip_v1_parity(inp, duration_s=duration, surface_mode=mode)
+    el = time.time() - t0
+    res = gates.validate(ref, cand, inp.surface.cell_area)
+    d = cand["depth_snapshots_m"][-1].astype(np.float64) - ref["depth_snapshots_m"][-1]
+    print(json.dumps({"mode": mode, "passed": bool(res["passed"]), "summary": res["summary"],
+                      "steps": [int(cand["report"]["step_count"]), int(ref["report"]["step_count"])],
+                      "gpu_run_s": el, "last_snapshot_max_abs_dh_m": float(np.max(np.abs(d))),
+                      "gates": res["gates"]}, default=float), flush=True)
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1]
     if cmd in ("oracle", "gpu"):
@@ -100,5 +140,9 @@ if __name__ == "__main__":
         run_oracle_steps(sys.argv[2])
     elif cmd == "gpu-steps":
         run_gpu_steps(sys.argv[2])
+    elif cmd == "oracle-gates":
+        run_oracle_gates(sys.argv[2], sys.argv[3])
+    elif cmd == "gpu-gates":
+        run_gpu_gates(sys.argv[2], sys.argv[3])
     else:
         raise SystemExit(__doc__)
