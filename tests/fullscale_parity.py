@@ -6,8 +6,8 @@ Not collected by pytest: the oracle half takes minutes of CPU time.
     python tests/fullscale_parity.py gpu    full_swe OUT.npz [T_END]   # GPU: same run, prints the comparison
     python tests/fullscale_parity.py oracle-steps OUT.npz              # CPU: 20 fixed-dt steps, node volumes per step
     python tests/fullscale_parity.py gpu-steps OUT.npz                 # GPU: per-step node-volume comparison
-    python tests/fullscale_parity.py oracle-gates full_swe OUT.npz     # CPU: the 1-hour event, snapshots
-    python tests/fullscale_parity.py gpu-gates full_swe OUT.npz        # GPU: same event, the eight gates
+    python tests/fullscale_parity.py oracle-gates full_swe OUT.npz [DURATION]   # CPU: the event (default 1 h)
+    python tests/fullscale_parity.py gpu-gates full_swe OUT.npz [DURATION]      # GPU: same event, the eight gates
 
 The gate run starts from a uniform 10 cm layer: the reference's flood-extent CSI is
 intersection / max(union, 1), which is 0 on a snapshot where neither run has a cell
@@ -141,8 +141,8 @@ if __name__ == "__main__":
     elif cmd == "gpu-steps":
         run_gpu_steps(sys.argv[2])
     elif cmd == "oracle-gates":
-        run_oracle_gates(sys.argv[2], sys.argv[3])
+        run_oracle_gates(sys.argv[2], sys.argv[3], float(sys.argv[4]) if len(sys.argv) > 4 else 3600.0)
     elif cmd == "gpu-gates":
-        run_gpu_gates(sys.argv[2], sys.argv[3])
+        run_gpu_gates(sys.argv[2], sys.argv[3], float(sys.argv[4]) if len(sys.argv) > 4 else 3600.0)
     else:
         raise SystemExit(__doc__)
