@@ -8,6 +8,7 @@ Not collected by pytest: the oracle half takes minutes of CPU time.
     python tests/fullscale_parity.py gpu-steps OUT.npz                 # GPU: per-step node-volume comparison
     python tests/fullscale_parity.py oracle-gates full_swe OUT.npz [DURATION]   # CPU: the event (default 1 h)
     python tests/fullscale_parity.py gpu-gates full_swe OUT.npz [DURATION]      # GPU: same event, the eight gates
+    python tests/fullscale_parity.py gpu-gates-fp64 full_swe [DURATION]         # GPU: float32 against FP64, the gates
 
 The gate run starts from a uniform 10 cm layer: the reference's flood-extent CSI is
 intersection / max(union, 1), which is 0 on a snapshot where neither run has a cell
@@ -131,6 +132,26 @@ def run_gpu_gates(mode, out, duration=3600.0):
                       "gates": res["gates"]}, default=float), flush=True)
 
 
+def run_gpu_gates_fp64(mode, duration=3600.0):
+    """The same event with an FP64 run of the same solver as the reference (the
+    kernels instantiated for double): float32 against FP64 through the eight gates."""
+    from floodhip import gates
+    from floodhip.runner import run_hip_v1_parity
+    inp = _gate_inputs()
+    # from the wet start every cell is wet, the smallest triangles included, and the
+    # CFL dt falls to about 0.05 s: the event needs more than the runner's default
+    # budget of 20,000 steps per hour
+    ref = run_hip_v1_parity(inp, duration_s=duration, surface_mode=mode, precision=64, max_steps_per_hour=400_000)
+    cand = run_hip_v1_parity(inp, duration_s=duration, surface_mode=mode, max_steps_per_hour=400_000)
+    res = gates.validate(ref, cand, inp.surface.cell_area)
+    d = cand["depth_snapshots_m"][-1].astype(np.float64) - ref["depth_snapshots_m"][-1]
+    print(json.dumps({"mode": mode, "reference": "FP64 run of the same solver", "passed": bool(res["passed"]),
+                      "summary": res["summary"],
+                      "steps": [int(cand["report"]["step_count"]), int(ref["report"]["step_count"])],
+                      "last_snapshot_max_abs_dh_m": float(np.max(np.abs(d))), "gates": res["gates"]},
+                     default=float), flush=True)
+
+
 if __name__ == "__main__":
     cmd = sys.argv[1]
     if cmd in ("oracle", "gpu"):
@@ -142,6 +163,8 @@ if __name__ == "__main__":
         run_gpu_steps(sys.argv[2])
     elif cmd == "oracle-gates":
         run_oracle_gates(sys.argv[2], sys.argv[3], float(sys.argv[4]) if len(sys.argv) > 4 else 3600.0)
+    elif cmd == "gpu-gates-fp64":
+        run_gpu_gates_fp64(sys.argv[2], float(sys.argv[3]) if len(sys.argv) > 3 else 3600.0)
     elif cmd == "gpu-gates":
         run_gpu_gates(sys.argv[2], sys.argv[3], float(sys.argv[4]) if len(sys.argv) > 4 else 3600.0)
     else:
