@@ -50,3 +50,33 @@ def test_max_abs_depth_check():
     a = np.zeros(10)
     assert gates.max_abs_depth_check(a, a + 5e-5)["passed"]
     assert not gates.max_abs_depth_check(a, a + 2e-4)["passed"]
+
+
+def test_ledger_gate_reads_v1_recharge_fields():
+    """A V1 run report names the recharge terms surface_recharge_* (parity_validation.py:21-22);
+    the candidate writes recharge_*.  A 3 % recharge mismatch against such a report trips gate 7."""
+    rng = np.random.default_rng(2)
+    d = np.abs(rng.normal(0.2, 0.1, (2, 500)))
+    area = np.full(500, 10.0)
+    v1 = _run(d)
+    rep = v1["report"]
+    for term in ("recharge_capture_m3", "recharge_loss_m3"):
+        rep["surface_" + term] = rep.pop(term)
+    ok = gates.validate(v1, _run(d), area)
+    assert ok["gates"]["source_sink_ledger_error_le_2pct"] and ok["ledger_terms_missing"] == []
+    assert ok["ledger_comparison"]["recharge_loss_m3"]["reference_field"] == "surface_recharge_loss_m3"
+    for term in ("recharge_capture_m3", "recharge_loss_m3"):
+        res = gates.validate(v1, _run(d, **{term: 103.0}), area)
+        assert not res["gates"]["source_sink_ledger_error_le_2pct"], term
+        assert abs(res["ledger_comparison"][term]["relative_error_fraction"] - 0.03) < 1e-12
+
+
+def test_ledger_gate_lists_missing_terms():
+    rng = np.random.default_rng(3)
+    d = np.abs(rng.normal(0.2, 0.1, (2, 500)))
+    ref = _run(d)
+    del ref["report"]["recharge_loss_m3"]
+    import pytest
+    with pytest.warns(UserWarning, match="6 of 7"):
+        res = gates.validate(ref, _run(d), np.full(500, 10.0))
+    assert res["ledger_terms_missing"] == ["recharge_loss_m3"]
