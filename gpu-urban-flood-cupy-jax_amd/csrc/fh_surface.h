@@ -28,10 +28,15 @@ namespace fh {
 constexpr int kGhostCopy = -1;
 constexpr int kGhostReflect = -2;
 constexpr int kGhostOutflow = -3;
-// stage boundary (code 3, and any code other than 0-2 and 4): a copy ghost in
-// the full step (the V1 step has no stage values), a wall in the local-inertial
-// step, which lets water out only at codes 4 and 1
+// stage boundary (code 3) of a member with stage values: the V1 Dirichlet stage
+// ghost (triangular_backend.py:1294-1327: depth max(stage - z_left, 0), zero
+// momentum, the left cell's bed), whose depth the host precomputes per edge.
+// A wall in the local-inertial step, which lets water out only at codes 4 and 1.
 constexpr int kGhostStage = -4;
+// code 3 without stage values (a copy ghost, as in v1_parity.py, which has no
+// stage values) and any code other than 0-4: a copy ghost in the full step, a
+// wall in the local-inertial step
+constexpr int kGhostOther = -5;
 
 // cell -> edge table entry: (edge_position << 1) | is_right_side, -1 none,
 // -2 in slot 0 marks an inactive (or padding) cell.
@@ -364,14 +369,15 @@ struct EdgeSeg {
 // cells of every edge (and of the orphan cells) into the step's minimum slot:
 // every cell counts at least once, and a cell counted twice gives the same value.
 // SH: shared-mesh batch; the edge tables hold member-local cells and are indexed
-// by the member-local edge.
+// by the member-local edge.  stage_h: the stage ghost depth of each edge position
+// (read at stage edges only).
 // ---------------------------------------------------------------------------
 template <typename real, int EPT, bool SH>
 __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
         const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
         const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc,
-        typename V4<real>::type* __restrict__ rec, real* __restrict__ recR) {
+        typename V4<real>::type* __restrict__ rec, real* __restrict__ recR, const real* __restrict__ stage_h) {
     typedef typename V4<real>::type V;
     __shared__ double lds[16];
     if ((int)blockIdx.x >= es.edge_blocks) {
@@ -439,6 +445,8 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         } else if (r == kGhostReflect) {
             const real nm = sl.y * nx + sl.z * ny;
             h_r = sl.x; hu_r = sl.y - real(2) * nm * nx; hv_r = sl.z - real(2) * nm * ny; z_r = sl.w;
+        } else if (r == kGhostStage) {
+            h_r = stage_h[e]; hu_r = real(0); hv_r = real(0); z_r = sl.w;
         } else {
             h_r = sl.x; hu_r = sl.y; hv_r = sl.z; z_r = sl.w;
         }

@@ -174,6 +174,11 @@ struct Member {
     std::vector<uint8_t> active;
     std::vector<real> rainm, f0, fc, hk;
     std::vector<float> rain_s, rain_e, rain_r;
+    // stage boundary: the staged positions of the code-3 edges and their caller
+    // edge ids; stage = one value per caller edge (empty: no stage values)
+    int64_t E_caller = 0;
+    std::vector<int> st_pos, st_edge;
+    std::vector<float> stage;
     // drainage
     int64_t N = 0, I = 0, L = 0, S = 0, R = 0, Rcv = 0, T = 0;
     std::vector<real> inv, As, dmax, oarea, ocoef, otw, ocap;
@@ -327,6 +332,7 @@ public:
     virtual void set_forcing(int64_t nr, const float* rs, const float* re, const float* rr, const float* m,
                              const float* f0, const float* fc, const float* k, int64_t nt, const float* et,
                              int64_t nrcv, const int32_t* rcv, const float* q) = 0;
+    virtual void set_boundary_stage(int64_t n, const float* stage) = 0;
     virtual void set_state(const float* h, const float* hu, const float* hv, const float* wet,
                            const float* vol, const float* q, const float* rs) = 0;
     virtual void get_state(float* h, float* hu, float* hv, float* wet, float* vol, float* q, float* rs) = 0;
@@ -448,7 +454,7 @@ public:
             if (cd == 0 || cd == 2) kind = fh::kGhostReflect;
             else if (!in && cd == 4) kind = fh::kGhostOutflow;
             else if (cd == 1) kind = fh::kGhostCopy;
-            else kind = fh::kGhostStage;
+            else kind = fh::kGhostOther;   // code 3 turns into kGhostStage when stage values are set
             ghost.push_back(e);
             ghost_kind.push_back(kind);
         }
@@ -461,6 +467,8 @@ public:
         std::vector<int32_t> pos_of((size_t)E_, -1);
         s.el.assign(s.E, 0); s.er.assign(s.E, 0);
         s.nx.assign(s.E, 0); s.ny.assign(s.E, 0); s.len.assign(s.E, 0);
+        s.E_caller = E_;
+        s.st_pos.clear(); s.st_edge.clear(); s.stage.clear();
         for (int64_t i = 0; i < s.E; ++i) {
             const bool gh = i >= s.E_int;
             const int64_t e = gh ? ghost[i - s.E_int] : interior[i];
@@ -468,6 +476,7 @@ public:
             s.el[i] = el[e];
             s.er[i] = gh ? ghost_kind[i - s.E_int] : er[e];
             s.nx[i] = (real)nx[e]; s.ny[i] = (real)ny[e]; s.len[i] = (real)len[e];
+            if (gh && er[e] < 0 && code[e] == 3) { s.st_pos.push_back((int)i); s.st_edge.push_back((int)e); }
         }
         // cell -> edge table: left-side entries (ascending original edge), then right-side
         std::vector<int> lent(C_ * 3, -1), rent(C_ * 3, -1);
@@ -583,6 +592,22 @@ public:
         s.rS = conv<real>(S_, R_); s.rcc = conv<real>(cc, R_); s.rrc = conv<real>(rc, R_);
         s.rstop = conv<real>(stop, R_);
         s.rstore.assign(R_, real(0));
+    }
+
+    // Stage values of the code-3 edges (TriangularSurfaceForcing.boundary_stage_m,
+    // triangular_backend.py:93-101): n == 1 is one stage for every edge, n == the
+    // caller's edge count one per edge, n == 0 (or NULL) removes them.
+    void set_boundary_stage(int64_t n, const float* stage) override {
+        Member<real>& s = cm();
+        if (!s.has_surface) throw ValueError{"fh_upload_surface must be called first"};
+        if (n < 0 || (n > 0 && !stage) || (n > 1 && n != s.E_caller))
+            throw ValueError{"boundary_stage_m must be one value or one value per edge"};
+        touch();
+        if (n == 0 || !stage) s.stage.clear();
+        else if (n == 1) s.stage.assign((size_t)s.E_caller, stage[0]);
+        else s.stage.assign(stage, stage + n);
+        const int kind = s.stage.empty() ? fh::kGhostOther : fh::kGhostStage;
+        for (int i : s.st_pos) s.er[(size_t)i] = kind;
     }
 
     void set_forcing(int64_t nr, const float* rs, const float* re, const float* rr, const float* m,
@@ -933,6 +958,23 @@ public:
             if (m.Rcv > 0) et_all.insert(et_all.end(), m.ext_t.begin(), m.ext_t.end());
         }
         d_el.set(el, s); d_er.set(er, s); d_nx.set(nx, s); d_ny.set(ny, s); d_len.set(len, s);
+        // stage ghost depth max(stage - z_left, 0) of every code-3 edge of the members
+        // with stage values, in the state precision, at the edge's device position
+        {
+            bool any = false;
+            for (const auto& m : mem) any = any || !m.stage.empty();
+            std::vector<real> gst(any ? (size_t)edges.total : 0, real(0));
+            for (int i = 0; any && i < M; ++i) {
+                const Member<real>& m = mem[order[i]];
+                for (size_t k = 0; k < m.stage.size() && k < m.st_pos.size(); ++k) {
+                    const int e = m.st_pos[k];
+                    const real d = (real)m.stage[(size_t)m.st_edge[k]] - m.bed[(size_t)m.el[(size_t)e]];
+                    const int64_t q = edges.base[i] + (m.epos.empty() ? e : m.epos[(size_t)e]);
+                    gst[(size_t)q] = (d != d || d > real(0)) ? d : real(0);
+                }
+            }
+            d_stage_h.set(gst, s);
+        }
         d_rec.alloc(std::max(edges.total, 1)); d_recR.alloc(std::max(edges.total, 1));
         d_ce0.set(ce0, s); d_ce1.set(ce1, s); d_ce2.set(ce2, s);
         if (li) {
@@ -1591,7 +1633,7 @@ public:
         const unsigned grid = (unsigned)(edges.blocks + orphans.blocks);
 #define FH_FLUX(EPT_, SH_)                                                                                    \
         go(fh::k_flux<real, EPT_, SH_>, grid, bs, ctl, kp, es, par, d_el.p, d_er.p, d_nx.p, d_ny.p, d_len.p, \
-           d_state.p, d_lc.p, d_rec.p, d_recR.p)
+           d_state.p, d_lc.p, d_rec.p, d_recR.p, d_stage_h.p)
 #define FH_FLUX_S(EPT_) if (shared) FH_FLUX(EPT_, true); else FH_FLUX(EPT_, false)
         if (ept == 4) { FH_FLUX_S(4); }
         else if (ept == 2) { FH_FLUX_S(2); }
@@ -2008,6 +2050,7 @@ public:
     DevArray<unsigned long long> d_cfl;   // CFL minimum slots [member][parity][slot] (fh::cfl_slot)
     std::vector<unsigned long long> cfl_inf;
     DevArray<real> d_nx, d_ny, d_len, d_recR, d_wet, d_area, d_n2, d_lc, d_rainm, d_f0, d_fc, d_k;
+    DevArray<real> d_stage_h;   // stage ghost depth per edge position (code-3 edges with stage values)
     DevArray<unsigned short> d_cls;   // source class per cell (when use_classes)
     DevArray<V> d_cls_tab;            // class -> {n^2, m, f0, fc}
     DevArray<real> d_cls_hk;          // class -> k
@@ -2206,6 +2249,11 @@ int fh_set_forcing(fh_ctx* ctx, int64_t n_intervals, const float* rain_start_s, 
     FH_GUARD(ctx, ctx->c.eng->set_forcing(n_intervals, rain_start_s, rain_end_s, rain_rate_mps, rainfall_multiplier,
                                           horton_f0_mps, horton_fc_mps, horton_k_per_s, n_times, external_times_s,
                                           n_receivers, external_receiver_cell, external_q_m3ps));
+    return FH_OK;
+}
+
+int fh_set_boundary_stage(fh_ctx* ctx, int64_t n_values, const float* stage_m) {
+    FH_GUARD(ctx, ctx->c.eng->set_boundary_stage(n_values, stage_m));
     return FH_OK;
 }
 

@@ -62,6 +62,7 @@ SIGNATURES = {
     "fh_upload_pipe": (C.c_int, [_ctx, _i64, _i, _i, _f, _f, _f, _f, _f, _f, _i, _f, _f, _f, _f, _f, _f]),
     "fh_upload_recharge": (C.c_int, [_ctx, _i64, _i, _f, _f, _f, _f]),
     "fh_set_forcing": (C.c_int, [_ctx, _i64, _f, _f, _f, _f, _f, _f, _f, _i64, _f, _i64, _i, _f]),
+    "fh_set_boundary_stage": (C.c_int, [_ctx, _i64, _f]),
     "fh_set_state": (C.c_int, [_ctx, _f, _f, _f, _f, _f, _f, _f]),
     "fh_get_state": (C.c_int, [_ctx, _f, _f, _f, _f, _f, _f, _f]),
     "fh_get_state_f64": (C.c_int, [_ctx, _d, _d, _d]),

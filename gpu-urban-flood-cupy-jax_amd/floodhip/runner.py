@@ -41,11 +41,12 @@ def output_targets(duration_s: float, interval_s: float = 3600.0) -> np.ndarray:
 
 def run_hip_v1_parity(inputs: LoadedParityInputs, *, duration_s: float, max_steps_per_hour: int = 20_000,
                       snapshot_interval_s: float = 3600.0, solver: HipParitySolver | None = None,
-                      keep_final_state: bool = True, **solver_opts) -> dict:
+                      keep_final_state: bool = True, boundary_stage_m=None, **solver_opts) -> dict:
     if duration_s <= 0:
         raise ConfigurationError("Parity duration must be positive.")
     s = solver or HipParitySolver(inputs.surface, inputs.coupling, inputs.pipe, inputs.recharge,
-                                  inputs.forcing, inputs.initial_state, **solver_opts)
+                                  inputs.forcing, inputs.initial_state, boundary_stage_m=boundary_stage_m,
+                                  **solver_opts)
     area = np.asarray(inputs.surface.cell_area, np.float64)
     initial_surface = float(np.sum(np.asarray(inputs.initial_state.h, np.float64) * area))
     initial_node = float(np.sum(np.asarray(inputs.initial_state.node_volume, np.float64)))

@@ -110,7 +110,7 @@ class HipParitySolver:
     def __init__(self, surface: SurfaceTopology | None, coupling: CouplingTopology | None = None,
                  pipe: PipeTopology | None = None, recharge: RechargeTopology | None = None,
                  forcing: ParityForcing | None = None, state: ParityState | None = None,
-                 options: SolverOptions | None = None, members: int = 1, **kw):
+                 options: SolverOptions | None = None, members: int = 1, boundary_stage_m=None, **kw):
         self.lib = _lib.load()
         opts = (options or SolverOptions())._replace(**kw)
         self.options = opts
@@ -149,7 +149,7 @@ class HipParitySolver:
                        for _ in range(self.members)]
         self._member = 0
         if surface is not None:
-            self.load_member(0, surface, coupling, pipe, recharge, forcing, state)
+            self.load_member(0, surface, coupling, pipe, recharge, forcing, state, boundary_stage_m)
 
     # per-member sizes of the selected member
     n_cells = property(lambda self: self._sizes[self._member]["n_cells"])
@@ -164,9 +164,12 @@ class HipParitySolver:
         self._member = int(member)
         return self
 
-    def load_member(self, member, surface, coupling=None, pipe=None, recharge=None, forcing=None, state=None):
+    def load_member(self, member, surface, coupling=None, pipe=None, recharge=None, forcing=None, state=None,
+                    boundary_stage_m=None):
         self.select(member)
         self.upload_surface(surface)
+        if boundary_stage_m is not None:
+            self.set_boundary_stage(boundary_stage_m)
         if coupling is not None and np.asarray(coupling.node_storage_area).shape[0] > 0:
             self.upload_coupling(coupling)
             self.upload_pipe(pipe if pipe is not None else empty_pipe(self.n_nodes))
@@ -267,6 +270,17 @@ class HipParitySolver:
             self._ctx, R, a.i(r.source_cell), a.f(r.storage_capacity), a.f(r.capture_capacity),
             a.f(r.recharge_capacity), a.f(r.stop_depth)), "upload_recharge")
         self._sizes[self._member]["n_recharge"] = R
+
+    def set_boundary_stage(self, boundary_stage_m):
+        """Stage values of the code-3 edges (TriangularSurfaceForcing.boundary_stage_m,
+        triangular_backend.py:93-101): a scalar or one value per edge; None removes
+        them (code 3 then copies the left state, as v1_parity.py does)."""
+        if boundary_stage_m is None:
+            self._check(self.lib.fh_set_boundary_stage(self._ctx, 0, None), "set_boundary_stage")
+            return
+        a = _Args()
+        v = np.atleast_1d(np.asarray(boundary_stage_m, np.float32)).reshape(-1)
+        self._check(self.lib.fh_set_boundary_stage(self._ctx, int(v.shape[0]), a.f(v)), "set_boundary_stage")
 
     def set_forcing(self, f: ParityForcing):
         a = _Args()

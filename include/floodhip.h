@@ -149,6 +149,18 @@ int fh_set_forcing(fh_ctx* ctx, int64_t n_intervals, const float* rain_start_s,
                    int64_t n_receivers, const int32_t* external_receiver_cell,
                    const float* external_q_m3ps);
 
+/* Stage values of the code-3 (Dirichlet stage) boundary edges of the selected
+ * member: TriangularSurfaceForcing.boundary_stage_m
+ * (gurugram_flood/kernels/triangular_backend.py:93-101), applied as the V1
+ * ghost of triangular_backend.py:1294-1327 / triangular_swe_cuda_kernels.py:86-93:
+ * depth max(stage - z_left, 0), zero momentum, the left cell's bed.
+ * n_values == 1: one stage for every edge; n_values == the n_edges given to
+ * fh_upload_surface: one per edge in the caller's order; 0 or NULL: none, and
+ * code 3 copies the left state as v1_parity.py's step does.  The local-inertial
+ * step treats code 3 as a wall either way.  A stage edge whose own (left) cell
+ * is inactive is dropped with every other edge of an inactive cell. */
+int fh_set_boundary_stage(fh_ctx* ctx, int64_t n_values, const float* stage_m);
+
 /* ParityState (v1_parity.py:80-87).  NULL leaves a field unchanged (set) or unread (get). */
 int fh_set_state(fh_ctx* ctx, const float* h, const float* hu, const float* hv,
                  const float* wet_time_s, const float* node_volume, const float* link_flow,

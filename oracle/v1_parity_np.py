@@ -219,8 +219,15 @@ def stable_timestep(state: ParityState, topology: SurfaceTopology, cfl=0.85, dty
     return F(np.min(local)) if local.size else F(np.inf)
 
 
-def surface_flux_step(state: ParityState, topology: SurfaceTopology, dt, dtype=np.float32):
+def surface_flux_step(state: ParityState, topology: SurfaceTopology, dt, dtype=np.float32,
+                      boundary_stage_m=None):
     """v1_parity.py:161-214 -- edge fluxes, scatter, cell update, Manning friction.
+
+    ``boundary_stage_m`` (a scalar or one value per edge) turns code-3 edges into
+    the V1 Dirichlet stage ghost of triangular_backend.py:1294-1327 (CUDA form
+    triangular_swe_cuda_kernels.py:86-93): depth max(stage - z_left, 0), zero
+    momentum, the left cell's bed.  None keeps v1_parity.py's ghost, which copies
+    the left state at code 3 (v1_parity.py:182-186 has no stage values).
 
     Returns (state, boundary_net, boundary_in, boundary_out).
     """
@@ -250,9 +257,16 @@ def surface_flux_step(state: ParityState, topology: SurfaceTopology, dt, dtype=n
     hu_reflect = hu_l - F(2) * normal_momentum * nx
     hv_reflect = hv_l - F(2) * normal_momentum * ny
     reflective = (code == BOUNDARY_CLOSED) | (code == BOUNDARY_REFLECTIVE)
-    h_r = np.where(active_interior, h_rb, h_l)
-    hu_r = np.where(active_interior, hu_rb, np.where(reflective, hu_reflect, hu_l))
-    hv_r = np.where(active_interior, hv_rb, np.where(reflective, hv_reflect, hv_l))
+    h_ghost, hu_ghost, hv_ghost = h_l, np.where(reflective, hu_reflect, hu_l), np.where(reflective, hv_reflect, hv_l)
+    if boundary_stage_m is not None:
+        stage = np.broadcast_to(np.asarray(boundary_stage_m, dtype=dtype), code.shape)
+        at_stage = code == BOUNDARY_STAGE
+        h_ghost = np.where(at_stage, np.maximum(stage - z_l, zero), h_ghost)
+        hu_ghost = np.where(at_stage, zero, hu_ghost)
+        hv_ghost = np.where(at_stage, zero, hv_ghost)
+    h_r = np.where(active_interior, h_rb, h_ghost)
+    hu_r = np.where(active_interior, hu_rb, hu_ghost)
+    hv_r = np.where(active_interior, hv_rb, hv_ghost)
     z_r = np.where(active_interior, z_rb, z_l)
     flux_l, flux_r = _hydrostatic_flux_pair(
         h_l, hu_l, hv_l, z_l, h_r, hu_r, hv_r, z_r, nx, ny, dtype)
@@ -557,12 +571,14 @@ def apply_recharge(state, surface, recharge, dt, dtype=np.float32):
 def advance_parity_step(state, surface, coupling, pipe, recharge, *, rainfall_rate_mps,
                         rainfall_multiplier, horton_f0_mps, horton_fc_mps, horton_k_per_s,
                         external_receiver_cell, external_q_m3ps, dt, dtype=np.float32,
-                        max_speed=5.0, surface_mode="full_swe"):
+                        max_speed=5.0, surface_mode="full_swe", boundary_stage_m=None):
     """v1_parity.py:430-469 -- one step in the V1 process order.
 
     ``surface_mode="local_inertial"`` swaps the surface operator for the
     reference's local-inertial step (engine.py:253-256, in oracle/local_inertial_np.py);
-    the sources, clamp, drainage and ledger are the same.
+    the sources, clamp, drainage and ledger are the same.  ``boundary_stage_m``
+    gives code-3 edges their stage ghost (surface_flux_step; the local-inertial
+    step treats code 3 as a wall either way).
     Returns (state, diagnostics) with diagnostics a dict over LEDGER_FIELDS (float64).
     """
     F = _f(dtype)
@@ -571,7 +587,7 @@ def advance_parity_step(state, surface, coupling, pipe, recharge, *, rainfall_ra
         from oracle import local_inertial_np
         state, bnet, bin_, bout = local_inertial_np.surface_step(state, surface, dt, dtype)
     elif surface_mode == "full_swe":
-        state, bnet, bin_, bout = surface_flux_step(state, surface, dt, dtype)
+        state, bnet, bin_, bout = surface_flux_step(state, surface, dt, dtype, boundary_stage_m)
     else:
         raise ValueError(f"unknown surface_mode {surface_mode!r}")
     state, rain, infiltration = apply_rainfall_horton(
@@ -650,7 +666,8 @@ def choose_dt(state, surface, clock: OracleClock, final_time_s, *, cfl=0.85, int
 
 def advance_to_time(state, clock: OracleClock, ledger, target_time_s, final_time_s,
                     surface, coupling, pipe, recharge, forcing, *, max_steps=20000,
-                    fixed_dt=None, dtype=np.float32, surface_mode="full_swe", **dt_kw):
+                    fixed_dt=None, dtype=np.float32, surface_mode="full_swe", boundary_stage_m=None,
+                    **dt_kw):
     """parity_runner.py:203-234 -- step while t < target (and the step budget lasts).
 
     The clock and ledger are kept in float64 (the FP64 clock/ledger the MI355X
@@ -669,7 +686,7 @@ def advance_to_time(state, clock: OracleClock, ledger, target_time_s, final_time
             horton_f0_mps=forcing.horton_f0_mps, horton_fc_mps=forcing.horton_fc_mps,
             horton_k_per_s=forcing.horton_k_per_s,
             external_receiver_cell=forcing.external_receiver_cell, external_q_m3ps=q,
-            dt=dt, dtype=dtype, surface_mode=surface_mode)
+            dt=dt, dtype=dtype, surface_mode=surface_mode, boundary_stage_m=boundary_stage_m)
         for k, v in diag.items():
             ledger[k] = ledger.get(k, 0.0) + v
         clock = OracleClock(clock.time_s + float(dt), evaluated, clock.global_step + 1)

@@ -20,7 +20,8 @@ def to_oracle(inputs, dtype=np.float32):
     return s, c, p, r, st, f
 
 
-def oracle_step(inputs_o, state, dt, rain=0.0, q=None, dtype=np.float32, surface_mode="full_swe"):
+def oracle_step(inputs_o, state, dt, rain=0.0, q=None, dtype=np.float32, surface_mode="full_swe",
+                boundary_stage_m=None):
     s, c, p, r, _, f = inputs_o
     if q is None:
         q = np.zeros(np.asarray(f.external_receiver_cell).shape[0], dtype)
@@ -28,7 +29,7 @@ def oracle_step(inputs_o, state, dt, rain=0.0, q=None, dtype=np.float32, surface
         state, s, c, p, r, rainfall_rate_mps=rain, rainfall_multiplier=f.rainfall_multiplier,
         horton_f0_mps=f.horton_f0_mps, horton_fc_mps=f.horton_fc_mps, horton_k_per_s=f.horton_k_per_s,
         external_receiver_cell=f.external_receiver_cell, external_q_m3ps=q, dt=dt, dtype=dtype,
-        surface_mode=surface_mode)
+        surface_mode=surface_mode, boundary_stage_m=boundary_stage_m)
 
 
 def basin(n=12, extent=120.0, code="closed", bed=None, h0=None, manning=0.0, active=None, seed=0):
