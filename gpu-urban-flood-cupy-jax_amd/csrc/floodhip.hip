@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <memory>
 #include <numeric>
@@ -2105,6 +2106,68 @@ struct fh_ctx {
     Ctx c;
 };
 
+// ---------------------------------------------------------------------------
+// Allocation-free CFL minimum (fh_stable_timestep_noalloc).  The same per-cell
+// CFL time and minimum as fh_stable_timestep, on caller-owned device arrays; the
+// per-block minima go to `scratch`, a file-scope device array (each GPU the
+// library runs on holds its own copy until the process ends), instead of the
+// context's CFL slots.  Host code never passes the array's name to a kernel: it
+// looks the array's device address up once per device (scratch_address) and
+// passes that.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocBlocks = 1024;     // block minima: 8 KiB per device
+constexpr int kNoAllocThreads = 256;
+constexpr int kNoAllocCellsPerBlock = 2048;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocBlocks];
+
+std::mutex g_scratch_mu;
+double* g_scratch_addr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the first
+// call there (hipGetSymbolAddress, with that device current); nullptr and the
+// HIP status in *err on failure.
+double* scratch_address(int device, hipError_t* err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) { *err = e; return nullptr; }
+        g_scratch_addr[device] = static_cast<double*>(p);
+    }
+    return g_scratch_addr[device];
+}
+
+// Block minimum of the CFL time over a grid-stride range of cells into part[block].
+__global__ void __launch_bounds__(kNoAllocThreads) k_cfl_noalloc(const float* __restrict__ h,
+        const float* __restrict__ hu, const float* __restrict__ hv, const float* __restrict__ lc, int64_t n,
+        fh::KP<float> p, double* __restrict__ part) {
+    __shared__ double lds[16];
+    float mn = INFINITY;
+    for (int64_t c = (int64_t)blockIdx.x * kNoAllocThreads + threadIdx.x; c < n;
+         c += (int64_t)gridDim.x * kNoAllocThreads) {
+        const float4 st = make_float4(h[c], hu[c], hv[c], 0.0f);
+        const float local = fh::cell_cfl<float>(p, st, lc[c]);
+        mn = local < mn ? local : mn;
+    }
+    const double v = fh::block_min((double)mn, lds);
+    if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+// Minimum of the nparts block minima (all written by the launch before on the
+// same stream) into *out; +inf when there are none.
+__global__ void __launch_bounds__(kNoAllocThreads) k_cfl_noalloc_final(const double* __restrict__ part,
+                                                                      int nparts, double* __restrict__ out) {
+    __shared__ double lds[16];
+    double mn = INFINITY;
+    for (int i = threadIdx.x; i < nparts; i += kNoAllocThreads) {
+        const double x = part[i];
+        mn = x < mn ? x : mn;
+    }
+    mn = fh::block_min(mn, lds);
+    if (threadIdx.x == 0) *out = mn;
+}
+
 #define FH_GUARD(ctx, body)                                                        \
     do {                                                                           \
         if (!(ctx)) { g_error = "NULL context"; return FH_ERR_VALUE; }             \
@@ -2325,6 +2388,46 @@ int fh_dry_weather_step(fh_ctx* ctx, double dt, const float* node_inflow_m3ps, c
 int fh_stable_timestep(fh_ctx* ctx, double* dt_out) {
     if (!dt_out) return FH_ERR_VALUE;
     FH_GUARD(ctx, *dt_out = ctx->c.eng->stable_timestep());
+    return FH_OK;
+}
+
+int fh_stable_timestep_noalloc(const float* h, const float* hu, const float* hv,
+                               const float* characteristic_length, int64_t n_cells, double cfl,
+                               double* dt_out, void* stream) {
+    if (n_cells < 0 || n_cells > FH_NOALLOC_MAX_CELLS) {
+        g_error = "fh_stable_timestep_noalloc: n_cells must be in [0, 2^31 - 1]";
+        return FH_ERR_VALUE;
+    }
+    if (!dt_out || (n_cells > 0 && (!h || !hu || !hv || !characteristic_length))) {
+        g_error = "fh_stable_timestep_noalloc: NULL device array";
+        return FH_ERR_VALUE;
+    }
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess || device < 0 || device >= kMaxDevices) {
+        g_error = std::string("fh_stable_timestep_noalloc: no usable current device: ") + hipGetErrorString(e);
+        return FH_ERR_UNAVAILABLE;
+    }
+    double* part = scratch_address(device, &e);
+    if (!part) {
+        g_error = std::string("fh_stable_timestep_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e);
+        return FH_ERR_RUNTIME;
+    }
+    fh::KP<float> p{};
+    p.g = (float)9.80665;
+    p.eps = (float)1.0e-5;
+    p.cfl = (float)cfl;
+    p.tiny = (float)1.0e-12;
+    const int64_t want = (n_cells + kNoAllocCellsPerBlock - 1) / kNoAllocCellsPerBlock;
+    const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(kNoAllocBlocks, want));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    k_cfl_noalloc<<<blocks, kNoAllocThreads, 0, s>>>(h, hu, hv, characteristic_length, n_cells, p, part);
+    k_cfl_noalloc_final<<<1, kNoAllocThreads, 0, s>>>(part, blocks, dt_out);
+    e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_error = std::string("fh_stable_timestep_noalloc: launch failed: ") + hipGetErrorString(e);
+        return FH_ERR_RUNTIME;
+    }
     return FH_OK;
 }
 

@@ -73,6 +73,8 @@ SIGNATURES = {
     "fh_advance_to": (C.c_int, [_ctx, C.c_double, C.c_double, _i64, C.POINTER(FhClock), _d]),
     "fh_step": (C.c_int, [_ctx, C.c_double, C.c_double, _f, _d]),
     "fh_stable_timestep": (C.c_int, [_ctx, _d]),
+    "fh_stable_timestep_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _i64, C.c_double,
+                                             C.c_void_p, C.c_void_p]),
     "fh_profile_kernels": (C.c_int, [_ctx, C.c_int32, C.c_int32, _i, C.c_char_p, _d, _d,
                                      C.POINTER(C.c_int64)]),
     "fh_device_bytes": (C.c_int64, [_ctx]),

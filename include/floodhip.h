@@ -193,6 +193,27 @@ int fh_step(fh_ctx* ctx, double dt, double rain_rate_mps, const float* external_
 /* stable_timestep (v1_parity.py:153-158) on the resident state. */
 int fh_stable_timestep(fh_ctx* ctx, double* dt_out);
 
+/* Allocation-free stable_timestep.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The CFL minimum of
+ * fh_stable_timestep (v1_parity.py:153-158: min over cells of
+ * cfl * Lc / (|u| + sqrt(g max(h, 1e-5))), non-finite values skipped, +inf for
+ * no cells), on caller-owned DEVICE arrays h, hu, hv and characteristic_length
+ * (float32, n_cells each); the result (a float32 value, as double) is written to
+ * the DEVICE address dt_out.  Enqueued on `stream` (a hipStream_t, NULL = the
+ * null stream) without synchronizing; calls no allocating routine.  Its block
+ * minima go to a fixed 8 KiB device array per GPU (1,024 doubles, held until the
+ * process ends), written before they are read in every call, so no state
+ * carries between calls; calls that overlap on one GPU (other streams or host
+ * threads) race on that array.  The first call on a device looks the array's
+ * address up: make it outside a graph capture.  n_cells must be at most
+ * FH_NOALLOC_MAX_CELLS; larger values return FH_ERR_VALUE (message via
+ * fh_global_error). */
+#define FH_NOALLOC_MAX_CELLS 2147483647LL
+int fh_stable_timestep_noalloc(const float* h, const float* hu, const float* hv,
+                               const float* characteristic_length, int64_t n_cells, double cfl,
+                               double* dt_out, void* stream);
+
 /* Sewer-only dry-weather flow, step_triangular_pipe_dry_weather_network
  * (gurugram_flood/kernels/triangular_backend.py:833-952): node inflow (m^3/s per
  * node in the caller's order, NULL = none), the link update and outfalls of the
