@@ -338,6 +338,7 @@ public:
                            const float* vol, const float* q, const float* rs) = 0;
     virtual void get_state(float* h, float* hu, float* hv, float* wet, float* vol, float* q, float* rs) = 0;
     virtual void get_state_f64(double* h, double* hu, double* hv) = 0;
+    virtual void get_drainage_f64(double* vol, double* q) = 0;
     virtual void set_clock(double t, double cached, int64_t step) = 0;
     virtual void get_clock(fh_clock* c) = 0;
     virtual void set_ledger(const double* l) = 0;
@@ -734,6 +735,18 @@ public:
             if (hu) hu[c] = (double)st[q].y;
             if (hv) hv[c] = (double)st[q].z;
         }
+    }
+
+    void get_drainage_f64(double* vol, double* q) override {
+        ensure_packed();
+        Member<real>& s = cm();
+        const int mp = pos_of_member(cur);
+        std::vector<real> v((size_t)s.N), qq((size_t)s.L);
+        if (s.N) d_V.get(v.data(), nodes.base[mp], s.N, stream);
+        if (s.L) d_Q.get(qq.data(), links.base[mp], s.L, stream);
+        FH_CHECK(hipStreamSynchronize(stream));
+        if (vol) for (int64_t n = 0; n < s.N; ++n) vol[n] = (double)v[(size_t)s.np((int)n)];
+        if (q) for (int64_t l = 0; l < s.L; ++l) q[l] = (double)qq[(size_t)s.lp((int)l)];
     }
 
     // ------------------------------------------------------------------ control blocks
@@ -2334,6 +2347,11 @@ int fh_get_state(fh_ctx* ctx, float* h, float* hu, float* hv, float* wet_time_s,
 
 int fh_get_state_f64(fh_ctx* ctx, double* h, double* hu, double* hv) {
     FH_GUARD(ctx, ctx->c.eng->get_state_f64(h, hu, hv));
+    return FH_OK;
+}
+
+int fh_get_drainage_f64(fh_ctx* ctx, double* node_volume, double* link_flow) {
+    FH_GUARD(ctx, ctx->c.eng->get_drainage_f64(node_volume, link_flow));
     return FH_OK;
 }
 

@@ -66,6 +66,7 @@ SIGNATURES = {
     "fh_set_state": (C.c_int, [_ctx, _f, _f, _f, _f, _f, _f, _f]),
     "fh_get_state": (C.c_int, [_ctx, _f, _f, _f, _f, _f, _f, _f]),
     "fh_get_state_f64": (C.c_int, [_ctx, _d, _d, _d]),
+    "fh_get_drainage_f64": (C.c_int, [_ctx, _d, _d]),
     "fh_set_clock": (C.c_int, [_ctx, C.c_double, C.c_double, _i64]),
     "fh_get_clock": (C.c_int, [_ctx, C.POINTER(FhClock)]),
     "fh_set_ledger": (C.c_int, [_ctx, _d]),

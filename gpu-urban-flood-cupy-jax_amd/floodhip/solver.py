@@ -328,6 +328,14 @@ class HipParitySolver:
         self._check(self.lib.fh_get_state_f64(self._ctx, P(h), P(hu), P(hv)), "get_state_f64")
         return h, hu, hv
 
+    def get_drainage_f64(self):
+        """Node volumes and link flows as float64 (the state itself when precision=64)."""
+        v = np.empty(self.n_nodes, np.float64)
+        q = np.empty(self.n_links, np.float64)
+        P = lambda x: x.ctypes.data_as(C.POINTER(C.c_double)) if x.size else None  # noqa: E731
+        self._check(self.lib.fh_get_drainage_f64(self._ctx, P(v), P(q)), "get_drainage_f64")
+        return v, q
+
     def set_clock(self, time_s=0.0, cached_dt_s=1.0, global_step=0):
         self._check(self.lib.fh_set_clock(self._ctx, float(time_s), float(cached_dt_s), int(global_step)),
                     "set_clock")

@@ -169,6 +169,9 @@ int fh_get_state(fh_ctx* ctx, float* h, float* hu, float* hv, float* wet_time_s,
                  float* node_volume, float* link_flow, float* recharge_storage);
 /* FP64 views of the cell state (meaningful when precision == 64). */
 int fh_get_state_f64(fh_ctx* ctx, double* h, double* hu, double* hv);
+/* FP64 views of the node volumes and link flows (meaningful when precision == 64);
+ * NULL leaves a field unread. */
+int fh_get_drainage_f64(fh_ctx* ctx, double* node_volume, double* link_flow);
 
 /* Clock and ledger (parity_runner.py:54-61 AdvanceCarry).  The ledger is 13
  * FP64 terms in MassLedger order, kept on the device. */

@@ -1,7 +1,8 @@
 """The benchmark workloads at their full sizes on the GPU.
 
 * 981,880-triangle coupled production workload: 20 fixed 0.5 s steps, every step
-  against the CPU oracle (depth, node volume, link flow, ledger terms); the FP64
+  against the CPU oracle (depth, node volume, link flow, ledger terms); the
+  one-step error along 120 steps of the oracle's CFL trajectory; the FP64
   ledger's balance identity (parity_runner.py:337-345); finite, non-negative
   depths; bit-identical repeat runs, graph and eager.
 * 200,000-triangle dam break, 120 s, in FP32, compensated FP32 and FP64 against
@@ -84,6 +85,35 @@ def test_production_fixed_steps_match_oracle(production):
     _report("config2_fixed_steps", steps=20, **worst, balance_residual_ratio=residual,
             rain_m3=ledger.rainfall_input, capture_m3=ledger.surface_capture)
     assert abs(residual) <= 1e-6
+
+
+def test_production_one_step_error_along_the_oracle_trajectory(production):
+    """Along 120 steps of the oracle's CFL trajectory (the C restatement, all host
+    cores) the GPU restarts from the oracle's float32 state before every step and
+    takes the same step: the one-step error stays at transcendental rounding, at
+    outfall nodes too (free-running trajectories part at the outfall threshold)."""
+    from oracle import v1_parity_c as oc
+    inp = production
+    s_, c_, p_, r_, st, f_ = to_oracle(inp)
+    co = oc.COracle(s_, c_, p_, r_, f_)
+    outfall = np.asarray(inp.pipe.outfall_area) > 0
+    clock, ledger = orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger()
+    worst = {"dh": 0.0, "node": 0.0, "outfall_node": 0.0, "link": 0.0}
+    with _solver(inp) as s:
+        for _ in range(120):
+            before = st
+            st, c1, ledger = co.advance_to_time(st, clock, ledger, 3600.0, 3600.0, max_steps=1)
+            s.set_state(before)
+            s.step(c1.time_s - clock.time_s, orc.rain_rate(f_, clock.time_s))
+            got = s.get_state()
+            rel = np.abs(got.node_volume.astype(np.float64) - st.node_volume) / np.maximum(np.abs(st.node_volume), 1.0)
+            q = np.abs(got.link_flow.astype(np.float64) - st.link_flow) / np.maximum(np.abs(st.link_flow), 1e-2)
+            for k, v in (("dh", float(np.max(np.abs(got.h - st.h)))), ("node", float(rel.max())),
+                         ("outfall_node", float(rel[outfall].max())), ("link", float(q.max()))):
+                worst[k] = max(worst[k], v)
+            clock = c1
+    _report("config2_one_step_error", steps=120, t=clock.time_s, **worst)
+    assert worst["dh"] <= 1e-6 and worst["node"] <= 1e-5 and worst["outfall_node"] <= 1e-5 and worst["link"] <= 1e-3
 
 
 def test_production_policy_run_is_repeatable_and_closes_in_fp64(production):
