@@ -61,25 +61,42 @@ def algorithmic_bytes(kernel, E, C, compensated=False):
     return None
 
 
-def cpu_baseline(inputs, steps):
-    """The CPU oracle (NumPy, float32, one process) on a bounded sample of the same workload."""
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(inputs, steps, numpy_steps):
+    """The CPU oracle on a bounded sample of the same workload, from its initial
+    state under the CFL policy: the C restatement on every host thread OpenMP
+    gets (OMP_NUM_THREADS, else all cores), and the NumPy oracle in one process
+    beside it.  Returns (C-oracle rate, threads, seconds, NumPy rate, seconds)."""
     sys.path.insert(0, str(ROOT / "tests"))
     from helpers import to_oracle
+    from oracle import v1_parity_c as oc
     from oracle import v1_parity_np as orc
     io = to_oracle(inputs)
-    st = io[4]
-    dt = np.float32(0.25)
-    t0 = time.perf_counter()
-    for i in range(steps):
-        rate = orc.rain_rate(io[5], i * float(dt))
-        st, _ = orc.advance_parity_step(
-            st, io[0], io[1], io[2], io[3], rainfall_rate_mps=rate, rainfall_multiplier=io[5].rainfall_multiplier,
-            horton_f0_mps=io[5].horton_f0_mps, horton_fc_mps=io[5].horton_fc_mps,
-            horton_k_per_s=io[5].horton_k_per_s, external_receiver_cell=io[5].external_receiver_cell,
-            external_q_m3ps=np.zeros(0, np.float32), dt=dt)
-    el = time.perf_counter() - t0
     C = inputs.surface.cell_area.shape[0]
-    return C * steps / el, el
+    threads = oc.set_threads(0)
+    co = oc.COracle(*io[:4], io[5])
+    t0 = time.perf_counter()
+    _, clock, _ = co.advance_to_time(io[4], orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger(), 3600.0, 3600.0,
+                                     max_steps=steps)
+    el = time.perf_counter() - t0
+    rate = C * clock.global_step / el
+    np_rate, np_el = None, 0.0
+    if numpy_steps > 0:
+        t0 = time.perf_counter()
+        _, c2, _ = orc.advance_to_time(io[4], orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger(), 3600.0, 3600.0,
+                                       *io[:4], io[5], max_steps=numpy_steps)
+        np_el = time.perf_counter() - t0
+        np_rate = C * c2.global_step / np_el
+    return rate, threads, el, np_rate, np_el
 
 
 def _ensemble_rate(members, steps, warmup, opts):
@@ -146,6 +163,28 @@ def batch_leg(base, args, plain_rate):
             "vs_fp64": {"fixed_dt_s": fp64["fixed_dt_s"], "steps": fp64["steps"], "worst_member": fp64["worst"]}}
 
 
+def wet_regime_leg(inp, args):
+    """The same workload from a uniform 10 cm start, where every cell is wet and the
+    CFL dt is about 0.05 s (the regime of the one-hour gate event): ms per step and
+    triangle-steps/s over a window of steps inside the event."""
+    from floodhip.solver import HipParitySolver
+    C = int(inp.surface.cell_area.shape[0])
+    st = inp.initial_state._replace(h=np.full(C, 0.1, np.float32))
+    with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, st,
+                         use_graph=not args.no_graph) as s:
+        s.advance_to(3600.0, 3600.0, max_steps=max(args.warmup, 1), raise_on_budget=False)
+        c0 = s.clock()
+        t0 = time.perf_counter()
+        c1, _ = s.advance_to(3600.0, 3600.0, max_steps=args.wet_steps, raise_on_budget=False)
+        el = time.perf_counter() - t0
+    steps = c1["global_step"] - c0["global_step"]
+    sim = c1["time_s"] - c0["time_s"]
+    log(f"wet regime: {steps} steps in {el:.3f}s: {1e3 * el / steps:.4f} ms/step, mean dt {sim / steps:.4f}s")
+    return {"workload": "the same 981,880-triangle coupled workload from a 10 cm wet start (every cell wet)",
+            "value": C * steps / el, "unit": "triangle-steps/s", "steps": steps, "ms_per_step": 1e3 * el / steps,
+            "mean_dt_s": sim / steps, "simulated_s_per_wall_s": sim / el}
+
+
 def local_inertial_leg(inp, args):
     """The reference's production surface mode (surface_step_mode: local_inertial,
     engine.py:219-256) on the same workload and network: the step's CFL pass, the
@@ -186,14 +225,15 @@ def batch_uneven_leg(args):
     opts = dict(compensated=True, **best)
     rate, el, steps, _ = _ensemble_rate(members, 64, 8, opts)
     seq = []
-    for m in members[:6]:
+    for m in members:
         r, _, _, _ = _ensemble_rate([m], 64, 8, dict(compensated=True))
         seq.append((int(m.surface.cell_area.shape[0]), r))
     # sequential plain: total work / total time if each member ran alone at its own rate
     seq_rate = sum(n for n, _ in seq) / sum(n / r for n, r in seq)
-    log(f"uneven batched {rate:.4e} vs sequential plain {seq_rate:.4e} (first 6 members)")
+    log(f"uneven batched {rate:.4e} vs sequential plain {seq_rate:.4e} (all {len(members)} members)")
     return {"members": len(members), "triangles": int(sum(sizes)), "value": rate, "unit": "triangle-steps/s",
-            "options": opts, "sequential_plain_value_sampled": seq_rate, "speedup_vs_sequential_plain": rate / seq_rate}
+            "options": opts, "sequential_plain_value": seq_rate, "sequential_plain_members": len(members),
+            "speedup_vs_sequential_plain": rate / seq_rate}
 
 
 def main():
@@ -201,7 +241,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=400, help="steps of the C oracle (all host threads)")
+    ap.add_argument("--numpy-steps", type=int, default=60, help="steps of the NumPy oracle (one process)")
+    ap.add_argument("--wet-steps", type=int, default=2000)
+    ap.add_argument("--no-wet", action="store_true", help="skip the wet-regime leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--compensated", action="store_true")
@@ -268,11 +311,23 @@ def main():
                                       if algorithmic_bytes(k["kernel"], E, C, args.compensated) else None)}
                for k in prof}
     s.close()
+    traffic = None if args.compensated else measured_traffic(dom["kernel"])
+    roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                "algorithmic_bytes": ab, "avg_us": 1e3 * dom["avg_ms"],
+                "byte_model": "each required array element once per step (flux 44E + 20C, update 20E + 72C)"}
+    if traffic:
+        # the same kernel's HBM bytes from the rocprofv3 counters (committed summary of
+        # an earlier run of this workload), over this run's kernel time
+        roofline.update({"traffic_source": str(TRAFFIC_FILE.relative_to(ROOT)) + " (rocprofv3 FETCH_SIZE and "
+                         "WRITE_SIZE per launch, scaled by the copy kernel's counts)",
+                         "frac_counter_bytes": traffic / (dom["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS})
     # the plain run's accuracy against its FP64 run (120 fixed 0.5 s steps)
     from floodhip.precision import ensemble_vs_fp64
     fp64 = ensemble_vs_fp64([inp], steps=120, dt=0.5, compensated=args.compensated, use_graph=True)
     log(f"plain vs FP64 (120 x 0.5 s): stored-volume rel {fp64['worst']['stored_volume_rel']:.3e}, "
         f"depth Linf {fp64['worst']['depth_linf_m']:.3e} m")
+    wet = None if args.no_wet else wet_regime_leg(inp, args)
     local_inertial = None if args.no_li else local_inertial_leg(inp, args)
     batched = None
     if not args.no_batch and args.batch_members > 0:
@@ -291,11 +346,15 @@ def main():
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU baseline: oracle, {args.cpu_steps} steps of the same workload")
-        v, cel = cpu_baseline(inp, args.cpu_steps)
-        cpu = {"value": v, "unit": "triangle-steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} steps of the 981,880-triangle coupled workload "
-                         f"(NumPy float32 oracle, one process, {cel:.1f}s)"}
-        log(f"CPU baseline {v:.4e} tri-steps/s")
+        v, threads, cel, npv, npel = cpu_baseline(inp, args.cpu_steps, args.numpy_steps)
+        cpu = {"value": v, "unit": "triangle-steps/s", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of the 981,880-triangle coupled workload from its initial state "
+                         f"under the CFL policy (C restatement of the oracle, float32, OpenMP, {cel:.1f}s)",
+               "host_cpu": _cpu_model(), "host_logical_cpus": os.cpu_count(),
+               "numpy_single_process": (None if npv is None else
+                                        {"value": npv, "unit": "triangle-steps/s", "cores": 1,
+                                         "sample": f"{args.numpy_steps} steps, NumPy float32 oracle ({npel:.1f}s)"})}
+        log(f"CPU baseline {v:.4e} tri-steps/s on {threads} threads; NumPy {npv}")
     out = {
         "metric": "triangle-steps/s + %HBM peak, 1 GPU; batched ensemble vs sequential plain",
         "value": tri_steps,
@@ -315,12 +374,11 @@ def main():
                    "links": int(inp.pipe.link_from.shape[0]), "inlets": int(inp.coupling.inlet_cell.shape[0]),
                    "seed": int(inp.report.get("seed", 0)), "compensated": bool(args.compensated),
                    "hip_graph": not args.no_graph, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if args.compensated else measured_traffic(dom["kernel"])},
+        "roofline": roofline,
         "kernels": kernels,
         "simulated_s_per_wall_s": (c1["time_s"] - c0["time_s"]) / el,
         "vs_fp64": {"fixed_dt_s": fp64["fixed_dt_s"], "steps": fp64["steps"], **fp64["members"][0]},
+        "wet_regime": wet,
         "local_inertial": local_inertial,
         "copy_probe": copy,
         "batched_shared_mesh": batched,

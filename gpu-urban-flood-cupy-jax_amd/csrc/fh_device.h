@@ -82,7 +82,7 @@ struct Ctl {
     double t_step;         // start time of the current step
     double ext_frac;       // external inflow interpolation fraction at t + dt/2
     int32_t ext_lower, ext_upper;
-    int32_t nonfinite;     // count of non-finite values found by k_finite
+    int32_t nonfinite;     // a non-finite depth or node volume was found (k_fold's check)
     int32_t li_missed;     // a local-inertial step evaluated the CFL condition without its CFL pass
     double cfl_min;        // result of the last CFL reduction
     double diag[kLedgerTerms];    // this step's ledger increments

@@ -930,6 +930,7 @@ __global__ void k_set_target(Ctl* ctls, int members, int parity, double target, 
     Clock& ck = ctl->clk[parity];
     ck.interval_step = 0;
     ck.active = (ck.t < target) && (0 < max_steps);
+    if (m == 0) ctl->nonfinite = 0;
 }
 
 __global__ void k_set_override(Ctl* ctls, int members, int on, double dt, double rain) {
@@ -939,29 +940,6 @@ __global__ void k_set_override(Ctl* ctls, int members, int on, double dt, double
     ctl->override_dt = on;
     ctl->override_dt_value = dt;
     ctl->override_rain = rain;
-}
-
-// Counts non-finite depths and node volumes over the whole batch
-// (parity_runner.py:319-322 checks); the count lands in ctls[0].nonfinite.
-template <typename real>
-__global__ void k_finite(Ctl* __restrict__ ctls, int n_cells, const typename V4<real>::type* __restrict__ state,
-                         int n_nodes, const real* __restrict__ vol, double* __restrict__ part) {
-    __shared__ double lds[16];
-    double bad = 0.0;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n_cells + n_nodes; c += gridDim.x * blockDim.x) {
-        const real v = c < n_cells ? state[c].x : vol[c - n_cells];
-        if (!isfinite(v)) bad += 1.0;
-    }
-    double r[1] = {bad};
-    block_sum<1>(r, lds);
-    publish_partials<1>(part, blockIdx.x, r);
-    if (!arrive_last(&ctls[0].counters[C_FINITE], gridDim.x)) return;
-    double s[1];
-    sum_partials<1>(part, gridDim.x, s, lds);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&ctls[0].counters[C_FINITE], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ctls[0].nonfinite = (int)s[0];
-    }
 }
 
 // Ledger accumulation slots of one reducing kernel: the member's block range
@@ -979,32 +957,68 @@ struct FoldTables {
     int ngroups;
 };
 
-// k_fold: one block per member sums the member's accumulation slots (fixed
-// order), writes the sums as the member's diagnostics, adds them to its FP64
-// ledger and clears the slots.
-__global__ void __launch_bounds__(256) k_fold(Ctl* __restrict__ ctls, FoldTables f) {
+// Sum of one member's slots of one group (N terms each) into v (thread 0), in a
+// fixed order, then the slots cleared.  The slot loads carry no store between
+// them, so each thread's loads of an unrolled stretch issue together.
+constexpr int kFoldThreads = 1024;
+template <int N>
+__device__ __forceinline__ void fold_group(double* __restrict__ acc, int b0, int nb, double (&v)[4], double* lds) {
+    double s[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) s[k] = 0.0;
+    const double* src = acc + (size_t)b0 * N;
+#pragma unroll 4
+    for (int i = threadIdx.x; i < nb; i += kFoldThreads) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) s[k] += src[(size_t)i * N + k];
+    }
+    block_sum<N>(s, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = s[k];
+    }
+    for (int i = threadIdx.x; i < nb * N; i += kFoldThreads) acc[(size_t)b0 * N + i] = 0.0;
+}
+
+// k_fold: block (member m, group g) sums the member's accumulation slots of
+// group g (fixed order), writes the sums as the member's diagnostics of the
+// group's ledger terms, adds them to the member's FP64 ledger and clears the
+// slots.  The groups hold disjoint terms; the boundary net term is the first
+// group's in - out.  Blocks from f.ngroups * members on check the batch's depths
+// and node volumes for non-finite values (parity_runner.py:319-322 checks),
+// one item per thread; every thread that finds one stores 1 in ctls[0].nonfinite.
+template <typename real>
+__global__ void __launch_bounds__(kFoldThreads) k_fold(Ctl* __restrict__ ctls, FoldTables f, int members,
+        int n_cells, const typename V4<real>::type* __restrict__ state, int n_nodes, const real* __restrict__ vol) {
     __shared__ double lds[16 * 4];
-    const int m = blockIdx.x;
+    const int blk = blockIdx.x;
+    if (blk >= f.ngroups * members) {
+        const int i = (blk - f.ngroups * members) * kFoldThreads + threadIdx.x;
+        if (i >= n_cells + n_nodes) return;
+        const real v = i < n_cells ? state[i].x : vol[i - n_cells];
+        if (!isfinite(v)) ctls[0].nonfinite = 1;
+        return;
+    }
+    const int m = blk / f.ngroups, gi = blk % f.ngroups;
     Ctl* ctl = ctls + m;
-    double d[kLedgerTerms];
-    for (int k = 0; k < kLedgerTerms; ++k) d[k] = 0.0;
-    for (int gi = 0; gi < f.ngroups; ++gi) {
-        const FoldGroup& g = f.g[gi];
-        const int b0 = g.blk0[m], nb = g.nblk[m];
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-            double* slot = g.acc + (size_t)(b0 + i) * g.n;
-            for (int k = 0; k < g.n; ++k) { v[k] += slot[k]; slot[k] = 0.0; }
-        }
-        block_sum<4>(v, lds);
-        if (threadIdx.x == 0)
-            for (int k = 0; k < g.n; ++k) d[g.terms[k]] = v[k];
+    const FoldGroup& g = f.g[gi];
+    const int b0 = g.blk0[m], nb = g.nblk[m];
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    switch (g.n) {
+        case 1: fold_group<1>(g.acc, b0, nb, v, lds); break;
+        case 2: fold_group<2>(g.acc, b0, nb, v, lds); break;
+        case 3: fold_group<3>(g.acc, b0, nb, v, lds); break;
+        default: fold_group<4>(g.acc, b0, nb, v, lds); break;
     }
     if (threadIdx.x == 0) {
-        d[L_BNET] = d[L_BIN] - d[L_BOUT];
-        for (int k = 0; k < kLedgerTerms; ++k) {
-            ctl->diag[k] = d[k];
-            ctl->ledger[k] += d[k];
+        for (int k = 0; k < g.n; ++k) {
+            ctl->diag[g.terms[k]] = v[k];
+            ctl->ledger[g.terms[k]] += v[k];
+        }
+        if (gi == 0) {
+            const double bnet = v[2] - v[3];   // the update group's terms: rain, infiltration, in, out
+            ctl->diag[L_BNET] = bnet;
+            ctl->ledger[L_BNET] += bnet;
         }
     }
 }

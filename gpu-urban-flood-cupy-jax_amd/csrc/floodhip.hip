@@ -652,6 +652,7 @@ public:
         Member<real>& s = cm();
         if (!s.has_surface) throw ValueError{"fh_upload_surface must be called first"};
         if (packed && !dirty) {
+            finalize_if_pending();
             // write the member's slices on the device
             const int cb = cells.base[pos_of_member(cur)];
             std::vector<V> st((size_t)s.C);
@@ -700,6 +701,7 @@ public:
 
     void get_state(float* h, float* hu, float* hv, float* wet, float* vol, float* q, float* rs) override {
         ensure_packed();
+        finalize_if_pending();
         Member<real>& s = cm();
         const int mp = pos_of_member(cur);
         const int cb = cells.base[mp];
@@ -725,6 +727,7 @@ public:
 
     void get_state_f64(double* h, double* hu, double* hv) override {
         ensure_packed();
+        finalize_if_pending();
         Member<real>& s = cm();
         std::vector<V> st((size_t)s.C);
         d_state.get(st.data(), cells.base[pos_of_member(cur)], s.C, stream);
@@ -753,6 +756,7 @@ public:
     void pull_ctl() {
         FH_CHECK(hipMemcpyAsync(ctl_h, ctl, sizeof(fh::Ctl) * M, hipMemcpyDeviceToHost, stream));
         FH_CHECK(hipStreamSynchronize(stream));
+        ctl_fresh = true;
     }
     void push_ctl() {
         FH_CHECK(hipMemcpyAsync(ctl, ctl_h, sizeof(fh::Ctl) * M, hipMemcpyHostToDevice, stream));
@@ -760,6 +764,7 @@ public:
         FH_CHECK(hipMemcpyAsync(d_cfl.p, cfl_inf.data(), cfl_inf.size() * sizeof(unsigned long long),
                                 hipMemcpyHostToDevice, stream));
         FH_CHECK(hipStreamSynchronize(stream));
+        ctl_fresh = true;
     }
     // ctl_h is indexed by member id (not pack position)
     void set_clock(double t, double cached, int64_t step) override {
@@ -775,7 +780,7 @@ public:
         push_ctl();
     }
     void get_clock(fh_clock* c) override {
-        pull_ctl();
+        if (!ctl_fresh) pull_ctl();
         const fh::Clock& x = ctl_h[cur].clk[par];
         c->time_s = x.t;
         c->cached_dt_s = x.cached_cfl;
@@ -790,7 +795,7 @@ public:
         push_ctl();
     }
     void get_ledger(double* l) override {
-        pull_ctl();
+        if (!ctl_fresh) pull_ctl();
         for (int k = 0; k < fh::kLedgerTerms; ++k) l[k] = ctl_h[cur].ledger[k];
     }
 
@@ -798,6 +803,7 @@ public:
     int pos_of_member(int m) const { return member_pos[(size_t)m]; }
 
     void pull_all() {
+        finalize_if_pending();
         // copy device state back into every member's host staging
         for (int m = 0; m < M; ++m) {
             Member<real>& s = mem[(size_t)m];
@@ -830,6 +836,8 @@ public:
 
     void pack() {
         drop_graph();
+        ctl_fresh = false;
+        wet_pending = false;
         for (size_t i = 0; i < mem.size(); ++i) {
             Member<real>& m = mem[i];
             if (P.cell_order == 1 && m.C > 0) {
@@ -1058,7 +1066,6 @@ public:
             d_ocells.set(oc, s);
             seg_orph.set(orphans, order, s);
         }
-        d_part_fin.alloc(1024);
         pack_drainage(icells, rcells, xcells, area);
         // control blocks: per member, unchanged
         FH_CHECK(hipStreamSynchronize(s));
@@ -1719,20 +1726,28 @@ public:
         launch_update();
         launch_drainage([](const char*, double, auto&& fn) { fn(); });
         par ^= 1;
+        ctl_fresh = false;
+        wet_pending = true;
     }
 
-    // Two captured step graphs: the full chunk and a short one for the last
-    // steps before a target, so an advance ends with few no-op steps.
+    // Captured step graphs of the full-step path, one per (steps, parity of the
+    // first step), for the chunk size K and every power of two below it: an
+    // advance of n steps is n / K chunks of K plus the binary decomposition of the
+    // rest, so a step budget below K takes at most log2(K) + 1 launches and no
+    // no-op steps.  All are captured at the first advance (capture records the
+    // launches and runs nothing).
     struct StepGraph {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int steps = 0;
     };
-    StepGraph graphs[2][2];   // [full / short][parity of the first step]
+    std::map<int, StepGraph> graphs;   // key (steps << 1) | parity
     // single-scenario local-inertial graphs, keyed by (full / short, parity, step
     // phase mod the CFL interval): only the steps that can evaluate launch the CFL pass
     std::map<int64_t, StepGraph> li_graphs;
     int64_t host_step = 0;    // the member's step count at the next chunk (phase-keyed graphs)
+    bool ctl_fresh = false;   // ctl_h holds the device control blocks (no launch since the last copy)
+    bool wet_pending = false; // the last executed step's wet-time increment is not applied yet
 
     bool li_phased() const { return li && M == 1 && P.use_graph; }
 
@@ -1743,10 +1758,45 @@ public:
     }
 
     void drop_graph() {
-        for (auto& row : graphs)
-            for (auto& g : row) destroy(g);
+        for (auto& kv : graphs) destroy(kv.second);
+        graphs.clear();
         for (auto& kv : li_graphs) destroy(kv.second);
         li_graphs.clear();
+    }
+
+    // Chunk sizes of the full-step graphs: K, then the powers of two below K.
+    std::vector<int> chunk_sizes() const {
+        const int K = std::max(1, P.steps_per_launch);
+        std::vector<int> v{K};
+        int q = 1;
+        while (q * 2 < K) q *= 2;
+        for (; q >= 1; q /= 2)
+            if (q < K) v.push_back(q);
+        return v;
+    }
+
+    void prebuild_graphs() {
+        if (!P.use_graph || li_phased() || !graphs.empty()) return;
+        for (int k : chunk_sizes())
+            for (int q = 0; q < 2; ++q) {
+                const int p0 = par;
+                par = q;
+                build_graph(graphs[(k << 1) | q], k);
+                par = p0;
+            }
+    }
+
+    // Launches n steps: chunks of K, then the largest chunk sizes that fit.
+    void launch_steps(int64_t n) {
+        const std::vector<int> sizes = chunk_sizes();
+        if (li_phased()) {
+            const int K = sizes[0], Kt = std::min(K, kTailSteps);
+            while (n >= K) { launch_chunk(K); n -= K; }
+            while (n > 0) { launch_chunk(Kt); n -= Kt; }
+            return;
+        }
+        for (int k : sizes)
+            while (n >= k) { launch_chunk(k); n -= k; }
     }
 
     // phase < 0: every step launches the CFL pass
@@ -1759,6 +1809,8 @@ public:
         FH_CHECK(hipStreamEndCapture(stream, &g.graph));
         par = p0;   // capture only recorded the steps
         FH_CHECK(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+        // move the graph's device-side setup out of its first launch
+        FH_CHECK(hipGraphUpload(g.exec, stream));
         g.steps = k;
     }
 
@@ -1775,17 +1827,19 @@ public:
             par ^= (k & 1);
             host_step += k;
         } else if (P.use_graph) {
-            StepGraph& g = graphs[k == std::max(1, P.steps_per_launch) ? 0 : 1][par];
+            StepGraph& g = graphs[(k << 1) | par];
             if (!g.exec || g.steps != k) build_graph(g, k);
             FH_CHECK(hipGraphLaunch(g.exec, stream));
             par ^= (k & 1);
         } else {
             for (int i = 0; i < k; ++i) launch_step();
         }
+        wet_pending = true;
     }
 
-    // Sum every member's ledger accumulation slots into its FP64 ledger.
-    void fold() {
+    // Sum every member's ledger accumulation slots into its FP64 ledger; with
+    // check, also flag non-finite depths and node volumes (one launch).
+    void fold(bool check = false) {
         fh::FoldTables f{};
         auto group = [&](int i, double* acc, const DevSeg& seg, int n, int t0, int t1, int t2) {
             f.g[i] = fh::FoldGroup{acc, seg.mem_blk0.p, seg.mem_nblk.p, n, {t0, t1, t2, 0}};
@@ -1797,20 +1851,32 @@ public:
         group(3, d_part_ret.p, seg_rx, 2, fh::L_RCAPTURE, fh::L_RLOSS, 0);
         group(4, d_part_ext.p, seg_xcl, 1, fh::L_EXTERNAL, 0, 0);
         f.ngroups = 5;
-        fh::k_fold<<<M, 256, 0, stream>>>(ctl, f);
+        launch_fold(f, check);
+    }
+
+    // the fold blocks, plus (check) blocks that flag a non-finite depth or node
+    // volume of the batch in ctls[0].nonfinite (cleared by k_set_target)
+    void launch_fold(const fh::FoldTables& f, bool check) {
+        const int64_t items = check ? (int64_t)cells.total + nodes.total : 0;
+        const int extra = (int)((items + fh::kFoldThreads - 1) / fh::kFoldThreads);
+        fh::k_fold<real><<<M * f.ngroups + extra, fh::kFoldThreads, 0, stream>>>(
+            ctl, f, M, check ? cells.total : 0, d_state.p, check ? nodes.total : 0, d_V.p);
+        ctl_fresh = false;
     }
 
     void finalize_pending() {
         fh::k_finalize_wet<real><<<std::max(cells.blocks, 1), bs, 0, stream>>>(ctl, kp, seg_cells.seg(), par,
                                                                                d_state.p, d_wet.p);
         fh::k_clear_pending<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, par);
+        ctl_fresh = false;
     }
 
-    bool check_finite() {
-        const int g = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (cells.total + 255) / 256));
-        fh::k_finite<real><<<g, 256, 0, stream>>>(ctl, cells.total, d_state.p, nodes.total, d_V.p, d_part_fin.p);
-        pull_ctl();
-        return ctl_h[0].nonfinite == 0;
+    // The wet-time increment of the last executed step is applied by the next
+    // step's update, or here, before the host reads or replaces the state.
+    void finalize_if_pending() {
+        if (!wet_pending || !packed || dirty) return;
+        finalize_pending();
+        wet_pending = false;
     }
 
     bool any_running(double target, int64_t max_steps) const {
@@ -1822,43 +1888,53 @@ public:
     int advance_to(double target, double final_t, int64_t max_steps) override {
         ensure_packed();
         if (max_steps <= 0) throw ValueError{"max_steps must be positive"};
+        prebuild_graphs();
         fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, par, target, final_t, max_steps);
         const int K = std::max(1, P.steps_per_launch);
-        pull_ctl();
-        while (any_running(target, max_steps)) {
+        if (!ctl_fresh) pull_ctl();
+        // the host's copy of the clocks is current; the step counters restart
+        for (int m = 0; m < M; ++m) ctl_h[m].clk[par].interval_step = 0;
+        bool done = false;
+        while (!done) {
             host_step = ctl_h[0].clk[par].step;
-            // several chunks when every member is far from the target (extra steps are no-ops)
-            double min_rem = 1e300;
-            int64_t budget_left = 0;
+            // steps the running members still need: their step budget, or the time
+            // left at the last dt (the dt cap before any step, which never
+            // overshoots).  A batch that covers every member's need is expected to
+            // end the advance, so the ledger fold and the finiteness check are
+            // queued ahead of its readback; a larger need is stepped in batches
+            // of three quarters of it, since the dt can grow at the next CFL
+            // evaluation and steps past the target are no-ops.
+            double need = 0.0;
+            bool exact = true;
             for (int m = 0; m < M; ++m) {
                 const fh::Clock& ck = ctl_h[m].clk[par];
                 if (!(ck.t < target && ck.interval_step < max_steps)) continue;
-                const double dt_est = std::max(ctl_h[m].dt > 0 ? ctl_h[m].dt : 1e-3, 1e-6);
-                min_rem = std::min(min_rem, (target - ck.t) / dt_est);
-                budget_left = std::max<int64_t>(budget_left, max_steps - ck.interval_step);
+                const double left = (double)(max_steps - ck.interval_step);
+                const double dt_last = ctl_h[m].dt > 0 ? ctl_h[m].dt : (kp.fixed_dt > 0.0 ? kp.fixed_dt : P.max_dt);
+                const double by_time = std::ceil((target - ck.t) / std::max(dt_last, 1e-9));
+                if (by_time < left) exact = false;
+                need = std::max(need, std::min(left, by_time));
             }
-            const int Kt = std::min(K, kTailSteps);
-            const double rem = std::min(min_rem, (double)budget_left);
-            if (rem < K && Kt < K) {
-                // close to the target or the step budget: short chunks, so the advance
-                // ends with at most Kt - 1 no-op steps
-                int n = (int)std::min<double>(8.0, std::ceil(std::max(rem, 1.0) / Kt));
-                n = (int)std::max<int64_t>(1, std::min<int64_t>(n, (budget_left + Kt - 1) / Kt));
-                for (int i = 0; i < n; ++i) launch_chunk(Kt);
+            if (need <= 0.0) break;
+            int64_t n;
+            bool last;
+            if (exact || need <= K) {
+                n = (int64_t)std::min(need, 64.0 * K);
+                last = need <= 64.0 * K;
             } else {
-                int chunks = (int)std::min<double>(8.0, std::max(1.0, std::floor(min_rem / K)));
-                chunks = (int)std::max<int64_t>(1, std::min<int64_t>(chunks, (budget_left + K - 1) / K));
-                for (int i = 0; i < chunks; ++i) launch_chunk(K);
+                n = (int64_t)std::min(16.0 * K, std::max((double)K, std::floor(0.75 * need)));
+                last = false;
             }
+            launch_steps(n);
+            if (last) fold(true);
             pull_ctl();
+            done = !any_running(target, max_steps);
+            if (done && !last) { fold(true); pull_ctl(); }
         }
-        fold();
-        finalize_pending();
-        const bool finite = check_finite();
         for (int m = 0; m < M; ++m)
             if (ctl_h[m].li_missed) throw std::runtime_error("local-inertial step evaluated the CFL condition "
                                                              "without its CFL pass");
-        if (!finite) return FH_ERR_SIMULATION;
+        if (ctl_h[0].nonfinite) return FH_ERR_SIMULATION;
         for (int m = 0; m < M; ++m)
             if (ctl_h[m].clk[par].t < target) return FH_ERR_BUDGET;
         return FH_OK;
@@ -1883,7 +1959,6 @@ public:
         launch_step();
         fh::k_set_override<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, 0, 0.0, 0.0);
         fold();
-        finalize_pending();
         dd.use_override = 0;
         pull_ctl();
         for (int m = 0; m < M; ++m) {
@@ -1897,6 +1972,7 @@ public:
 
     void dry_weather_step(double dt, const float* inflow, const float* tw, int cap, double* diag) override {
         ensure_packed();
+        ctl_fresh = false;
         if (M != 1) throw ValueError{"dry-weather stepping acts on single-scenario contexts"};
         if (!(dt > 0.0)) throw ValueError{"timestep_s must be positive."};
         const Member<real>& m = mem[0];
@@ -2005,6 +2081,8 @@ public:
         }
         fold();
         finalize_pending();
+        wet_pending = false;
+        ctl_fresh = false;
         FH_CHECK(hipStreamSynchronize(stream));
         for (const Pending& q : pending) {
             float ms = 0.f;
@@ -2070,7 +2148,7 @@ public:
     DevArray<real> d_cls_hk;          // class -> k
     bool use_classes = false;
     DevArray<V> d_rec, d_state, d_comp;
-    DevArray<double> d_part_upd, d_part_fin, d_part_inlet, d_part_node, d_part_ret,
+    DevArray<double> d_part_upd, d_part_inlet, d_part_node, d_part_ret,
         d_part_ext;
     DevArray<float> d_rain_s, d_rain_e, d_rain_r, d_ext_t, d_ext_q, d_ext_ov;
     // drainage

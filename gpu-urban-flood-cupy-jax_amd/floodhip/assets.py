@@ -58,16 +58,23 @@ HORTON_ARRAYS = ("horton_f0_mmhr", "horton_fc_mmhr", "horton_decay_per_hr", "bui
                  "impervious_fraction", "bcr_open_fraction")
 
 
-def _sha256(path: Path) -> str:
-    digest = hashlib.sha256()
-    with path.open("rb") as handle:
-        for block in iter(lambda: handle.read(1024 * 1024), b""):
-            digest.update(block)
-    return digest.hexdigest()
+SNAP_WARNING_M = 500.0          # external inflow points snapped farther than this are counted
+RECHARGE_STOP_DEPTH_M = 0.02    # stop depth of a recharge row that leaves it blank
+_TRUE_WORDS = ("1", "true", "yes", "y")
 
 
-def _enabled(value) -> bool:
-    return str(value).strip().lower() in {"1", "true", "yes", "y"}
+def _digest(path: Path) -> dict:
+    h = hashlib.sha256()
+    with path.open("rb") as f:
+        while chunk := f.read(1 << 20):
+            h.update(chunk)
+    return {"path": str(path), "sha256": h.hexdigest(), "size_bytes": path.stat().st_size}
+
+
+def _truthy(values) -> np.ndarray:
+    """Element-wise: the text is one of 1/true/yes/y, ignoring case and blanks."""
+    words = np.char.lower(np.char.strip(np.asarray(values, dtype=str)))
+    return np.isin(words, _TRUE_WORDS)
 
 
 def asset_paths(v1_root) -> dict[str, Path]:
@@ -76,57 +83,97 @@ def asset_paths(v1_root) -> dict[str, Path]:
             "recharge": root / RECHARGE_RELATIVE, "historical_boundary_inflow": root / BOUNDARY_INFLOW_RELATIVE}
 
 
+def _boundary_distribution(topology) -> dict[str, int]:
+    codes = topology["surface_boundary_code"][topology["surface_edge_right_cell"] < 0]
+    values, counts = np.unique(codes, return_counts=True)
+    return {str(int(v)): int(c) for v, c in zip(values, counts)}
+
+
+def _nonnegative_finite(a) -> bool:
+    a = np.asarray(a)
+    return bool(np.isfinite(a).all() and not (a < 0).any())
+
+
+# The topology contract of a parity run (parity_assets.py:47-104), one rule per
+# row: (applies in non-strict mode, problem found in the opened archive or None).
+_TOPOLOGY_RULES = (
+    (True, lambda t: (f"the topology archive lacks {sorted(set(EXPECTED_SHAPES) - set(t.files))}"
+                      if set(EXPECTED_SHAPES) - set(t.files) else None)),
+    (False, lambda t: next((f"production shape mismatch: {n} is {t[n].shape}, the production mesh has {s}"
+                            for n, s in EXPECTED_SHAPES.items() if t[n].shape != s), None)),
+    (False, lambda t: (None if _boundary_distribution(t) == PRODUCTION_BOUNDARY else
+                       f"exterior boundary codes {_boundary_distribution(t)} differ from the production "
+                       f"contract (34,390 outflow-only code-4 edges)")),
+    (True, lambda t: (None if _nonnegative_finite(t["surface_cell_area_m2"])
+                      and (t["surface_cell_area_m2"] > 0).all() else "cell areas must be positive and finite")),
+    (True, lambda t: next((f"{n} has negative or non-finite entries" for n in
+                           ("inlet_capture_capacity_m3ps", "link_flow_capacity_m3ps", "outfall_flow_capacity_m3ps")
+                           if not _nonnegative_finite(t[n])), None)),
+)
+
+
 def validate_v1_parity_assets(v1_root, *, strict: bool = True) -> dict:
-    """parity_assets.py:47-138 -- fail fast unless the parity assets are present and consistent."""
+    """The asset gate of a parity run (parity_assets.py:47-138): the four asset files
+    exist; the topology archive passes _TOPOLOGY_RULES (the production shapes and
+    boundary contract only when ``strict``); the Horton/BCR archive has its six
+    per-triangle arrays.  Returns the readiness report: topology summary, recharge
+    row counts and a sha256 per asset."""
     root = Path(v1_root).resolve()
     paths = asset_paths(root)
-    missing = [str(p) for p in paths.values() if not p.is_file()]
-    if missing:
-        raise DataFileMissingError("Missing V1 parity assets: " + ", ".join(missing))
-    with np.load(paths["topology"], allow_pickle=False) as topology:
-        absent = sorted(set(EXPECTED_SHAPES).difference(topology.files))
-        if absent:
-            raise ConfigurationError(f"V1 topology is missing parity arrays: {absent}")
-        if strict:
-            wrong = {n: {"expected": s, "actual": topology[n].shape} for n, s in EXPECTED_SHAPES.items()
-                     if topology[n].shape != s}
-            if wrong:
-                raise ConfigurationError(f"V1 topology shape mismatch: {wrong}")
-        right = topology["surface_edge_right_cell"]
-        codes, counts = np.unique(topology["surface_boundary_code"][right < 0], return_counts=True)
-        boundary = {str(int(c)): int(n) for c, n in zip(codes, counts)}
-        if strict and boundary != PRODUCTION_BOUNDARY:
-            raise ConfigurationError("V1 exterior boundary contract changed; expected 34,390 outflow-only edges "
-                                     f"(code 4), found {boundary}.")
-        active = topology["active_surface_mask"].astype(bool)
-        area = topology["surface_cell_area_m2"].astype(np.float64)
-        if np.any(area <= 0) or not np.isfinite(area).all():
-            raise ConfigurationError("V1 surface cell areas must be positive and finite.")
-        for name in ("inlet_capture_capacity_m3ps", "link_flow_capacity_m3ps", "outfall_flow_capacity_m3ps"):
-            values = topology[name]
-            if np.any(~np.isfinite(values)) or np.any(values < 0):
-                raise ConfigurationError(f"V1 {name} contains negative or non-finite values.")
-        summary = {"triangle_count": int(topology["triangles"].shape[0]), "active_triangle_count": int(active.sum()),
-                   "active_area_km2": float(area[active].sum() / 1e6), "edge_count": int(right.size),
-                   "exterior_boundary_distribution": boundary,
-                   "inlet_count": int(topology["inlet_cell_index"].size),
-                   "node_count": int(topology["node_storage_area_m2"].size),
-                   "link_count": int(topology["link_from_node_index"].size)}
-        n_cells = int(topology["triangles"].shape[0])
+    absent = [str(p) for p in paths.values() if not p.is_file()]
+    if absent:
+        raise DataFileMissingError("V1 parity assets not found: " + ", ".join(absent))
+    with np.load(paths["topology"], allow_pickle=False) as topo:
+        for applies_loosely, rule in _TOPOLOGY_RULES:
+            problem = rule(topo) if (strict or applies_loosely) else None
+            if problem:
+                raise ConfigurationError(f"V1 topology rejected: {problem}.")
+        active = topo["active_surface_mask"].astype(bool)
+        area = topo["surface_cell_area_m2"].astype(np.float64)
+        n_cells = int(topo["triangles"].shape[0])
+        summary = {"triangle_count": n_cells, "active_triangle_count": int(active.sum()),
+                   "active_area_km2": float(area[active].sum() / 1e6),
+                   "edge_count": int(topo["surface_edge_right_cell"].size),
+                   "exterior_boundary_distribution": _boundary_distribution(topo),
+                   "inlet_count": int(topo["inlet_cell_index"].size),
+                   "node_count": int(topo["node_storage_area_m2"].size),
+                   "link_count": int(topo["link_from_node_index"].size)}
     with np.load(paths["horton"], allow_pickle=False) as horton:
-        absent = sorted(set(HORTON_ARRAYS).difference(horton.files))
-        if absent:
-            raise ConfigurationError(f"V1 Horton/BCR file is missing arrays: {absent}")
-        if any(horton[n].shape != (n_cells,) for n in HORTON_ARRAYS):
-            raise ConfigurationError("Every V1 Horton/BCR array must have one value per triangle.")
-    with paths["recharge"].open(newline="", encoding="utf-8", errors="replace") as handle:
-        rows = list(csv.DictReader(handle))
+        lacking = [n for n in HORTON_ARRAYS if n not in horton.files]
+        if lacking:
+            raise ConfigurationError(f"V1 Horton/BCR archive lacks {lacking}.")
+        if {horton[n].shape for n in HORTON_ARRAYS} != {(n_cells,)}:
+            raise ConfigurationError("V1 Horton/BCR arrays must hold one value per triangle.")
+    table = _read_csv_columns(paths["recharge"])
+    enabled = _truthy(table.get("hydraulic_enabled", np.full(_rows(table), "")))
     return {"schema_version": 1, "ready": True, "source_root": str(root), "crs": "EPSG:32643",
             "reference_profile": "v1_exact_production_full_swe", "topology": summary,
-            "recharge": {"input_rows": len(rows), "enabled_rows": sum(_enabled(r.get("hydraulic_enabled", ""))
-                                                                     for r in rows)},
-            "assets": {name: {"path": str(p), "sha256": _sha256(p), "size_bytes": p.stat().st_size}
-                       for name, p in paths.items()}}
+            "recharge": {"input_rows": _rows(table), "enabled_rows": int(enabled.sum())},
+            "assets": {name: _digest(p) for name, p in paths.items()}}
+
+
+def _read_csv_columns(path: Path) -> dict[str, np.ndarray]:
+    """Every column of a CSV as text, blanks as empty strings."""
+    with path.open(newline="", encoding="utf-8", errors="replace") as f:
+        reader = csv.reader(f)
+        header = next(reader, [])
+        body = [row + [""] * (len(header) - len(row)) for row in reader]
+    return {name: np.asarray([row[i] for row in body], dtype=object).astype(str) for i, name in enumerate(header)}
+
+
+def _rows(table: dict) -> int:
+    return len(next(iter(table.values()))) if table else 0
+
+
+def _numbers(table: dict, name: str, default: float, rows) -> np.ndarray:
+    """Column ``name`` as float64 over ``rows``; blank cells and a missing column give ``default``."""
+    if name not in table:
+        return np.full(rows.sum(), default)
+    text = table[name][rows]
+    out = np.full(text.shape, default)
+    given = np.char.strip(text) != ""
+    out[given] = text[given].astype(np.float64)
+    return out
 
 
 def load_rainfall_csv(path) -> dict:
@@ -186,67 +233,65 @@ def load_rainfall_csv(path) -> dict:
     return event
 
 
-def _snap(active, centroids):
-    from scipy.spatial import cKDTree
-    index = np.flatnonzero(active)
-    return index, cKDTree(centroids[index])
+class _ActiveCells:
+    """Nearest active cell of a point, by cell centroid."""
+
+    def __init__(self, active, centroids):
+        from scipy.spatial import cKDTree
+        self.index = np.flatnonzero(active)
+        self.tree = cKDTree(np.asarray(centroids)[self.index])
+
+    def nearest(self, xy):
+        distance, k = self.tree.query(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+        return self.index[k], distance
 
 
 def _load_recharge(path: Path, active, centroids):
-    """parity_inputs.py:62-90 -- enabled rows; a source off the active mesh snaps to the nearest active centroid."""
-    with path.open(newline="", encoding="utf-8", errors="replace") as handle:
-        rows = [r for r in csv.DictReader(handle) if _enabled(r.get("hydraulic_enabled"))]
-    index, tree = _snap(active, centroids)
-    source, storage, capture, loss, stop = [], [], [], [], []
-    remapped = 0
-    for row in rows:
-        cell = int(float(row.get("source_cell_index") or -1))
-        if cell < 0 or cell >= len(active) or not active[cell]:
-            _, nearest = tree.query([float(row["x_m"]), float(row["y_m"])])
-            cell = int(index[nearest])
-            remapped += 1
-        source.append(cell)
-        storage.append(float(row.get("storage_capacity_m3") or 0.0))
-        capture.append(float(row.get("capture_capacity_m3ps") or 0.0))
-        loss.append(float(row.get("recharge_capacity_m3ps") or 0.0))
-        stop.append(float(row.get("stop_depth_m") or 0.02))
-    recharge = RechargeTopology(np.asarray(source, np.int32), np.asarray(storage, np.float32),
-                                np.asarray(capture, np.float32), np.asarray(loss, np.float32),
-                                np.asarray(stop, np.float32))
-    return recharge, {"row_count": len(rows), "remapped_source_count": remapped}
+    """Recharge rows of the exchange table (parity_inputs.py:65-91): enabled rows
+    only; a source cell that is blank, off the mesh or inactive is replaced by
+    the active cell whose centroid is nearest to the row's (x_m, y_m); blank
+    capacities are 0 and a blank stop depth is RECHARGE_STOP_DEPTH_M."""
+    table = _read_csv_columns(path)
+    n = _rows(table)
+    keep = _truthy(table["hydraulic_enabled"]) if "hydraulic_enabled" in table else np.zeros(n, bool)
+    cell = _numbers(table, "source_cell_index", -1.0, keep).astype(np.int64)
+    active = np.asarray(active, bool)
+    usable = (cell >= 0) & (cell < active.size)
+    usable[usable] = active[cell[usable]]
+    moved = ~usable
+    if moved.any():
+        xy = np.column_stack((_numbers(table, "x_m", np.nan, keep)[moved], _numbers(table, "y_m", np.nan, keep)[moved]))
+        cell[moved], _ = _ActiveCells(active, centroids).nearest(xy)
+    recharge = RechargeTopology(cell.astype(np.int32),
+                                _numbers(table, "storage_capacity_m3", 0.0, keep).astype(np.float32),
+                                _numbers(table, "capture_capacity_m3ps", 0.0, keep).astype(np.float32),
+                                _numbers(table, "recharge_capacity_m3ps", 0.0, keep).astype(np.float32),
+                                _numbers(table, "stop_depth_m", RECHARGE_STOP_DEPTH_M, keep).astype(np.float32))
+    return recharge, {"row_count": int(keep.sum()), "remapped_source_count": int(moved.sum())}
 
 
 def _external_forcing(path: Path, active, centroids):
-    """parity_inputs.py:93-127 -- receivers snapped to active centroids; one column per
-    (inflow id, receiver cell); rows at the same time summed."""
+    """The external boundary hydrographs (parity_inputs.py:94-127): each row's point
+    goes to its nearest active cell; every distinct (inflow id, cell) pair is one
+    receiver, in order of first appearance; the table's distinct times, ascending,
+    are the rows of q, and rows that share a receiver and a time add up."""
     import pandas as pd
     frame = pd.read_csv(path)
-    index, tree = _snap(active, centroids)
-    ids = frame["inflow_id"].astype(str).to_numpy()
-    x = pd.to_numeric(frame["x_utm43n_m"], errors="raise").to_numpy(float)
-    y = pd.to_numeric(frame["y_utm43n_m"], errors="raise").to_numpy(float)
-    distance, nearest = tree.query(np.column_stack((x, y)))
-    receiver_by_row = index[nearest]
-    time = pd.to_numeric(frame["time_s"], errors="raise").to_numpy(float)
-    q = pd.to_numeric(frame["q_m3ps"], errors="raise").to_numpy(float)
-    times = np.asarray(sorted(set(time)), dtype=np.float64)
-    at = {float(v): i for i, v in enumerate(times)}
-    key_of: dict[tuple[str, int], int] = {}
-    receivers: list[int] = []
-    columns: list[np.ndarray] = []
-    for inflow_id, cell, t, flow in zip(ids, receiver_by_row, time, q):
-        key = (str(inflow_id), int(cell))
-        if key not in key_of:
-            key_of[key] = len(receivers)
-            receivers.append(int(cell))
-            columns.append(np.zeros(len(times), np.float64))
-        columns[key_of[key]][at[float(t)]] += float(flow)
-    q_matrix = np.stack(columns, axis=1).astype(np.float32)
-    report = {"receiver_count": len(receivers), "maximum_snap_m": float(np.max(distance)),
-              "snap_warning_count": int(np.sum(distance > 500.0)),
-              "peak_total_m3ps": float(np.max(q_matrix.sum(axis=1))),
-              "hydrograph_volume_m3": float(np.trapezoid(q_matrix.sum(axis=1), times))}
-    return times.astype(np.float32), np.asarray(receivers, np.int32), q_matrix, report
+    num = {c: pd.to_numeric(frame[c], errors="raise").to_numpy(np.float64)
+           for c in ("x_utm43n_m", "y_utm43n_m", "time_s", "q_m3ps")}
+    cell, distance = _ActiveCells(active, centroids).nearest(np.column_stack((num["x_utm43n_m"],
+                                                                              num["y_utm43n_m"])))
+    keys = pd.MultiIndex.from_arrays([frame["inflow_id"].astype(str).to_numpy(), cell])
+    column, receivers = pd.factorize(keys)
+    times, row = np.unique(num["time_s"], return_inverse=True)
+    q = np.zeros((times.size, len(receivers)))
+    np.add.at(q, (row, column), num["q_m3ps"])
+    q = q.astype(np.float32)
+    total = q.sum(axis=1)
+    report = {"receiver_count": len(receivers), "maximum_snap_m": float(distance.max()),
+              "snap_warning_count": int((distance > SNAP_WARNING_M).sum()),
+              "peak_total_m3ps": float(total.max()), "hydrograph_volume_m3": float(np.trapezoid(total, times))}
+    return (times.astype(np.float32), np.asarray(receivers.get_level_values(1), np.int32), q, report)
 
 
 def load_parity_inputs(*, rainfall_csv, v1_root, strict: bool = True) -> LoadedParityInputs:

@@ -26,18 +26,16 @@ BOUNDARY_CODES = {"closed": 0, "transmissive": 1, "reflective": 2, "stage": 3, "
 
 
 def orient_triangles_ccw(nodes: np.ndarray, triangles: np.ndarray):
-    """triangular_backend.py:1815-1829."""
-    tri = np.array(triangles, dtype=np.int64, copy=True)
-    p0, p1, p2 = nodes[tri[:, 0]], nodes[tri[:, 1]], nodes[tri[:, 2]]
-    a, b = p1 - p0, p2 - p0
-    twice = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
-    cw = twice < 0.0
-    if np.any(cw):
-        tmp = tri[cw, 1].copy()
-        tri[cw, 1] = tri[cw, 2]
-        tri[cw, 2] = tmp
-        twice[cw] = -twice[cw]
-    return tri, 0.5 * twice
+    """Counter-clockwise vertex order and cell area (the reference's
+    _orient_triangles_ccw, triangular_backend.py:1815-1829): a triangle whose
+    signed area is negative has its last two vertices exchanged."""
+    tri = np.asarray(triangles, dtype=np.int64)
+    corner = nodes[tri]                                   # (C, 3, 2)
+    e1, e2 = corner[:, 1] - corner[:, 0], corner[:, 2] - corner[:, 0]
+    twice = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]
+    clockwise = twice < 0.0
+    oriented = np.where(clockwise[:, None], tri[:, (0, 2, 1)], tri)
+    return oriented, 0.5 * np.where(clockwise, -twice, twice)
 
 
 def build_edges(nodes: np.ndarray, tri: np.ndarray):
@@ -80,44 +78,58 @@ def build_edges(nodes: np.ndarray, tri: np.ndarray):
     return left, right, nx, ny, L.astype(np.float32), perimeter
 
 
+# (argument, expected shape given (nodes, triangles) counts, what it must be)
+_INPUT_SHAPES = (
+    ("nodes_xy", lambda nn, nc: (nn, 2), "an (n, 2) array of node coordinates"),
+    ("triangles", lambda nn, nc: (nc, 3), "an (n, 3) array of node indices"),
+    ("bed_elevation_m", lambda nn, nc: (nc,), "one value per triangle"),
+)
+
+
+def _per_cell(value, n_cells: int, default: float, what: str) -> np.ndarray:
+    """A scalar (or None: ``default``) broadcast to every cell, or one value per cell."""
+    v = np.asarray(default if value is None else value, dtype=np.float32)
+    if v.ndim == 0:
+        return np.full(n_cells, v, dtype=np.float32)
+    if v.shape != (n_cells,):
+        raise ValueError(f"{what} takes a scalar or one value per triangle, got shape {v.shape}.")
+    return v.copy()
+
+
 def build_surface_topology(nodes_xy, triangles, bed_elevation_m, manning_n=None,
-                           active_mask=None, boundary_mode="outflow") -> SurfaceTopology:
-    """Mirror of triangular_backend.py:244-326 (build_triangular_surface_topology)."""
+                           active_mask=None, boundary_mode="outflow"):
+    """The reference's surface topology from a node/triangle mesh
+    (build_triangular_surface_topology, triangular_backend.py:244-326): CCW
+    cells, one record per edge (build_edges), the boundary code on every edge
+    without a right cell, Manning n (default 0.035), the active mask, and
+    Lc = 2 A / perimeter.  An interior edge whose left cell is inactive and right
+    cell active is given to the active cell (left and right exchanged, normal
+    reversed), so every edge's left cell is active when either is.  Returns
+    (SurfaceTopology, oriented triangles)."""
     nodes = np.asarray(nodes_xy, dtype=np.float64)
     tri = np.asarray(triangles, dtype=np.int64)
     bed = np.asarray(bed_elevation_m, dtype=np.float32)
-    if nodes.ndim != 2 or nodes.shape[1] != 2:
-        raise ValueError("nodes_xy must have shape (node_count, 2).")
-    if tri.ndim != 2 or tri.shape[1] != 3:
-        raise ValueError("triangles must have shape (triangle_count, 3).")
-    if bed.shape != (tri.shape[0],):
-        raise ValueError("bed_elevation_m must have one value per triangle.")
-    if np.any(tri < 0) or np.any(tri >= nodes.shape[0]):
-        raise ValueError("triangles contain node indices outside nodes_xy.")
+    n_nodes = nodes.shape[0] if nodes.ndim == 2 else -1
+    n_cells = tri.shape[0] if tri.ndim == 2 else -1
+    for (name, shape_of, meaning), value in zip(_INPUT_SHAPES, (nodes, tri, bed)):
+        if value.shape != shape_of(n_nodes, n_cells):
+            raise ValueError(f"{name} must be {meaning}; got shape {value.shape}.")
+    if tri.size and (tri.min() < 0 or tri.max() >= n_nodes):
+        raise ValueError("triangles refer to nodes that nodes_xy does not have.")
     oriented, area = orient_triangles_ccw(nodes, tri)
     if np.any(area <= 1.0e-12):
-        raise ValueError("triangles contain degenerate or near-zero-area cells.")
+        raise ValueError("the mesh has degenerate (near-zero-area) triangles.")
     left, right, nx, ny, length, perimeter = build_edges(nodes, oriented)
-    code = np.full(left.shape, -1, dtype=np.int32)
-    code[right < 0] = BOUNDARY_CODES[boundary_mode] if isinstance(boundary_mode, str) else int(boundary_mode)
-    if manning_n is None:
-        manning = np.full(tri.shape[0], 0.035, dtype=np.float32)
-    elif np.isscalar(manning_n):
-        manning = np.full(tri.shape[0], float(manning_n), dtype=np.float32)
-    else:
-        manning = np.asarray(manning_n, dtype=np.float32)
-    if active_mask is None:
-        active = np.ones(tri.shape[0], dtype=bool)
-    else:
-        active = np.asarray(active_mask) > 0.5
-        interior = right >= 0
-        swap = interior & (~active[left]) & active[np.maximum(right, 0)]
-        if np.any(swap):
-            l0 = left.copy()
-            left[swap] = right[swap]
-            right[swap] = l0[swap]
-            nx[swap] *= -1.0
-            ny[swap] *= -1.0
+    code = np.where(right < 0, BOUNDARY_CODES[boundary_mode] if isinstance(boundary_mode, str)
+                    else int(boundary_mode), -1).astype(np.int32)
+    manning = _per_cell(manning_n, n_cells, 0.035, "manning_n")
+    active = np.ones(n_cells, dtype=bool) if active_mask is None else _per_cell(active_mask, n_cells, 1.0,
+                                                                                "active_mask") > 0.5
+    # hand interior edges of an inactive left cell to their active right cell
+    flip = (right >= 0) & ~active[left] & active[np.where(right >= 0, right, 0)]
+    left, right = np.where(flip, right, left).astype(np.int32), np.where(flip, left, right).astype(np.int32)
+    sign = np.where(flip, np.float32(-1.0), np.float32(1.0))
+    nx, ny = (nx * sign).astype(np.float32), (ny * sign).astype(np.float32)
     lc = 2.0 * area / np.maximum(perimeter, 1.0e-12)
     return SurfaceTopology(
         bed=bed, manning_n=manning, cell_area=area.astype(np.float32),
