@@ -25,10 +25,6 @@ enum LedgerTerm {
     L_OUTFALL, L_OVERFLOW, L_UNRETURNED, L_RCAPTURE, L_RLOSS
 };
 
-// Completion counters of the last-block reductions (one per reducing kernel).
-enum Counter { C_BEGIN = 0, C_FLUX, C_UPDATE, C_INLET, C_NODE, C_RETURN, C_EXTERNAL, C_FINITE,
-               C_COUNT = 16 };
-
 // Start-of-step control state of one member, double-buffered by step parity:
 // every block of a step's update kernel reads clk[p] and the update's first
 // block writes clk[1 - p] for the next step, so no block ever reads a clock
@@ -87,7 +83,6 @@ struct Ctl {
     double cfl_min;        // result of the last CFL reduction
     double diag[kLedgerTerms];    // this step's ledger increments
     double ledger[kLedgerTerms];  // cumulative FP64 ledger
-    unsigned counters[C_COUNT];
 };
 
 // Block -> member mapping of one index space (cells, edges, nodes, ...).  Each
@@ -283,42 +278,6 @@ __device__ __forceinline__ T block_min(T v, T* lds) {
     return v;
 }
 
-// Last-block-done: every block of the reduction publishes its partials, then
-// the block that arrives last (grid-wide counter) sums all partials in a fixed
-// order and commits them.  Returns true in the last block (all threads).
-//
-// The partials are written and read with agent-scope atomic accesses, which are
-// coherent across the XCDs' L2 caches, so no cache-wide fence is needed: thread
-// 0 only waits until its own partial stores have completed before it bumps the
-// counter.  (An agent-scope __threadfence here would write back the whole L2 of
-// the XCD once per block.)
-__device__ __forceinline__ bool arrive_last(unsigned* counter, unsigned nblocks) {
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_s_waitcnt(0);
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_last = (prev == nblocks - 1);
-    }
-    __syncthreads();
-    return s_last != 0;
-}
-
-// Sum `nparts` partial records of N doubles (stride N) with the whole block, in a
-// fixed order.  Result in out[] of thread 0.
-template <int N>
-__device__ __forceinline__ void sum_partials(const double* part, int nparts, double (&out)[N],
-                                             double* lds) {
-#pragma unroll
-    for (int k = 0; k < N; ++k) out[k] = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            out[k] += __hip_atomic_load(part + (size_t)i * N + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    block_sum<N>(out, lds);
-}
-
 // Adds one block's partials to the block's own accumulation slot.  Only this
 // block ever writes the slot and launches are stream-ordered, so the slots
 // need no atomics; k_fold sums them into the FP64 ledger in a fixed order.
@@ -348,16 +307,5 @@ struct SlotAcc {
         }
     }
 };
-
-// Publish one block's partial record (agent-scope stores: visible to the last
-// block wherever it runs, across XCD L2s).
-template <int N>
-__device__ __forceinline__ void publish_partials(double* part, int index, const double (&v)[N]) {
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k)
-            __hip_atomic_store(part + (size_t)index * N + k, v[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-}
 
 }  // namespace fh

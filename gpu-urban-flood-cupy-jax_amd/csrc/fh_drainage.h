@@ -141,22 +141,6 @@ constexpr int kInletThreads = 512;
 constexpr int kNodeBlock = 256;
 constexpr int kLinkThreads = 512;
 
-// commit helper used by the last block of each ledger kernel
-template <int NT>
-__device__ __forceinline__ void commit_terms(Ctl* ctl, int counter, const double* part, int nparts,
-                                             const int (&terms)[NT], double* lds) {
-    double s[NT];
-    sum_partials<NT>(part, nparts, s, lds);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&ctl->counters[counter], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            ctl->diag[terms[k]] = s[k];
-            ctl->ledger[terms[k]] += s[k];
-        }
-    }
-}
-
 // v1_parity.py:407-412 -- h[recv] += max(q, 0) dt / A, q interpolated at t + dt/2.
 template <typename real>
 __global__ void __launch_bounds__(256) k_external(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,

@@ -313,8 +313,8 @@ __device__ __forceinline__ void commit_plan(Ctl* ctl, int parity, const Clock& c
 
 // ---------------------------------------------------------------------------
 // k_cfl: CFL minimum over every cell of each member (stable_timestep,
-// v1_parity.py:153-158); wave shuffle -> LDS -> one atomic min per block, the
-// last-arriving block of the member stores it in cfl_min.
+// v1_parity.py:153-158); wave shuffle -> LDS -> one atomic min per block into
+// the member's slots, which k_cfl_final reduces into cfl_min.
 // ---------------------------------------------------------------------------
 struct CellRanges {
     Seg seg;                 // k_cfl blocks of each member
@@ -323,12 +323,11 @@ struct CellRanges {
 };
 
 template <typename real, bool LI>
-__global__ void __launch_bounds__(256) k_cfl(Ctl* __restrict__ ctls, KP<real> p, int parity,
+__global__ void __launch_bounds__(256) k_cfl(KP<real> p, int parity,
         const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc, CellRanges cr) {
     __shared__ double lds[16 * 2];
     const int b = blockIdx.x;
     const int m = cr.seg.blk_member[b];
-    Ctl* ctl = ctls + m;
     const int blk0 = cr.seg.mem_blk0[m], nblk = cr.seg.mem_nblk[m];
     real mn = real(INFINITY);
     const int c1 = cr.cell1[m];
@@ -336,20 +335,20 @@ __global__ void __launch_bounds__(256) k_cfl(Ctl* __restrict__ ctls, KP<real> p,
         const real local = cell_time<real, LI>(p, state[c], lc[c]);
         if (local < mn) mn = local;
     }
-    // the next step's slots (parity) hold +inf between steps and are left so
+    // the next step's slots (parity) hold +inf between steps; k_cfl_final reads
+    // them after this launch has completed and restores +inf
     commit_cfl_min(p.cfl_slots, m, parity, b - blk0, (double)mn, lds);
-    if (!arrive_last(&ctl->counters[C_BEGIN], nblk)) return;
-    if (threadIdx.x != 0) return;
-    __hip_atomic_store(&ctl->counters[C_BEGIN], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long bits = ~0ull;
-    for (int k = 0; k < kCflSlots; ++k) {
-        unsigned long long* sl = cfl_slot(p.cfl_slots, m, parity, k);
-        const unsigned long long x = __hip_atomic_load(sl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        bits = x < bits ? x : bits;
-        __hip_atomic_store(sl, (unsigned long long)__double_as_longlong(INFINITY), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-    }
-    ctl->cfl_min = __longlong_as_double((long long)bits);
+}
+
+// One 64-lane wave per member, launched after k_cfl on the same stream: the
+// launch boundary orders every block's slot update before these reads, so the
+// reduction needs no grid-wide completion counter.
+__global__ void __launch_bounds__(64) k_cfl_final(Ctl* __restrict__ ctls, unsigned long long* __restrict__ cfl,
+                                                  int parity) {
+    const int m = blockIdx.x;
+    const double v = cfl_slots_min(cfl, m, parity);
+    *cfl_slot(cfl, m, parity, (int)threadIdx.x) = (unsigned long long)__double_as_longlong(INFINITY);
+    if (threadIdx.x == 0) ctls[m].cfl_min = v;
 }
 
 // Edge index space: Seg, plus the blocks after the edge blocks that cover each

@@ -1621,11 +1621,12 @@ public:
     // CFL minimum over every cell of each member into cfl_min (stable_timestep)
     void launch_cfl() {
         if (li)
-            go(fh::k_cfl<real, true>, begin_l.blocks, 256, ctl, kp, par, d_state.p, d_lc.p,
+            go(fh::k_cfl<real, true>, begin_l.blocks, 256, kp, par, d_state.p, d_lc.p,
                fh::CellRanges{seg_begin.seg(), d_cell0.p, d_cell1.p});
         else
-            go(fh::k_cfl<real, false>, begin_l.blocks, 256, ctl, kp, par, d_state.p, d_lc.p,
+            go(fh::k_cfl<real, false>, begin_l.blocks, 256, kp, par, d_state.p, d_lc.p,
                fh::CellRanges{seg_begin.seg(), d_cell0.p, d_cell1.p});
+        go(fh::k_cfl_final, (unsigned)M, 64, ctl, kp.cfl_slots, par);
     }
     // local-inertial surface: the step's CFL pass, edge discharges, donor limiting
     void launch_li_cfl() {

@@ -458,31 +458,127 @@ class HipEnsemble(HipParitySolver):
 # functional mirrors of the reference entry points
 # ---------------------------------------------------------------------------
 
+class _ContextCache:
+    """Resident contexts of the functional entry points, so a caller that loops
+    over advance_parity_step / stable_timestep uploads its topology and forcing
+    once instead of on every call.
+
+    A context is found by the identity of every input array and the value of
+    every scalar option; the cache keeps those arrays alive (so their ids stay
+    theirs) and a digest of their bytes, recomputed on each call, so an array
+    changed in place gets a fresh context rather than a stale one.  The state is
+    uploaded and read back on every call (the functions are pure).  At most
+    ``slots`` contexts are kept (least recently used first out)."""
+
+    def __init__(self, slots: int = 2):
+        import threading
+        from collections import OrderedDict
+        self.slots = slots
+        self._lock = threading.Lock()
+        self._entries: "OrderedDict[tuple, tuple[list, int, HipParitySolver]]" = OrderedDict()
+
+    @staticmethod
+    def _leaves(x, out):
+        if isinstance(x, np.ndarray):
+            out.append(x)
+            return ("a", id(x))
+        if isinstance(x, (tuple, list)):
+            return tuple(_ContextCache._leaves(v, out) for v in x)
+        if isinstance(x, dict):
+            return tuple((k, _ContextCache._leaves(x[k], out)) for k in sorted(x))
+        if x is None or isinstance(x, (bool, int, float, str, np.generic)):
+            return x
+        arr = np.asarray(x)
+        out.append(arr)
+        return ("a", id(arr))
+
+    @staticmethod
+    def _digest(arrays) -> int:
+        try:
+            import xxhash
+            h = xxhash.xxh3_64()
+        except ImportError:  # pragma: no cover - xxhash ships with this image
+            import hashlib
+            h = hashlib.blake2b(digest_size=8)
+        for a in arrays:
+            h.update(f"{a.dtype.str}{a.shape}".encode())
+            if a.size:
+                h.update(np.ascontiguousarray(a).view(np.uint8).reshape(-1))
+        return int.from_bytes(h.digest()[:8], "little")
+
+    def get(self, inputs, build):
+        """The context for ``inputs`` (any nesting of arrays, tuples and scalars),
+        created by ``build()`` when none matches."""
+        arrays: list = []
+        key = self._leaves(inputs, arrays)
+        digest = self._digest(arrays)
+        with self._lock:
+            hit = self._entries.pop(key, None)
+            if hit is not None and hit[1] == digest and hit[2]._ctx:
+                self._entries[key] = hit
+                return hit[2]
+            if hit is not None:
+                hit[2].close()
+            solver = build()
+            self._entries[key] = (arrays, digest, solver)
+            while len(self._entries) > self.slots:
+                _, (_, _, old) = self._entries.popitem(last=False)
+                old.close()
+            return solver
+
+    def clear(self):
+        with self._lock:
+            for _, _, s in self._entries.values():
+                s.close()
+            self._entries.clear()
+
+
+_contexts = _ContextCache()
+
+
+def clear_context_cache():
+    """Release the GPU contexts kept by advance_parity_step / stable_timestep."""
+    _contexts.clear()
+
+
 def advance_parity_step(state: ParityState, surface: SurfaceTopology, coupling: CouplingTopology,
                         pipe: PipeTopology, recharge: RechargeTopology, *, rainfall_rate_mps,
                         rainfall_multiplier, horton_f0_mps, horton_fc_mps, horton_k_per_s,
                         external_receiver_cell, external_q_m3ps, dt, **opts):
-    """v1_parity.py:430-469 on the GPU: returns (state, StepDiagnostics)."""
+    """v1_parity.py:430-469 on the GPU: returns (state, StepDiagnostics).
+
+    The topology and per-cell forcing stay resident between calls with the same
+    arrays (see _ContextCache); each call uploads ``state``, takes one step of
+    ``dt`` and returns the new state."""
     n = np.asarray(surface.cell_area).shape[0]
     rcv = np.asarray(external_receiver_cell, np.int32).reshape(-1)
     q = np.asarray(external_q_m3ps, np.float32).reshape(-1)
-    forcing = ParityForcing(
-        np.zeros(0, np.float32), np.zeros(0, np.float32), np.zeros(0, np.float32),
-        np.broadcast_to(np.asarray(rainfall_multiplier, np.float32), (n,)),
-        np.broadcast_to(np.asarray(horton_f0_mps, np.float32), (n,)),
-        np.broadcast_to(np.asarray(horton_fc_mps, np.float32), (n,)),
-        np.broadcast_to(np.asarray(horton_k_per_s, np.float32), (n,)),
-        np.zeros(1 if rcv.size else 0, np.float32), rcv,
-        q.reshape(1, -1) if rcv.size else np.zeros((0, 0), np.float32))
-    with HipParitySolver(surface, coupling, pipe, recharge, forcing, state, **opts) as s:
-        diag = s.step(dt, float(rainfall_rate_mps), q if rcv.size else None)
-        return s.get_state(), diag
+
+    def build():
+        forcing = ParityForcing(
+            np.zeros(0, np.float32), np.zeros(0, np.float32), np.zeros(0, np.float32),
+            np.broadcast_to(np.asarray(rainfall_multiplier, np.float32), (n,)),
+            np.broadcast_to(np.asarray(horton_f0_mps, np.float32), (n,)),
+            np.broadcast_to(np.asarray(horton_fc_mps, np.float32), (n,)),
+            np.broadcast_to(np.asarray(horton_k_per_s, np.float32), (n,)),
+            np.zeros(1 if rcv.size else 0, np.float32), rcv,
+            q.reshape(1, -1) if rcv.size else np.zeros((0, 0), np.float32))
+        return HipParitySolver(surface, coupling, pipe, recharge, forcing, None, **opts)
+
+    s = _contexts.get(("step", surface, coupling, pipe, recharge, rainfall_multiplier, horton_f0_mps,
+                       horton_fc_mps, horton_k_per_s, external_receiver_cell, opts), build)
+    s.set_state(state)
+    diag = s.step(dt, float(rainfall_rate_mps), q if rcv.size else None)
+    return s.get_state(), diag
 
 
 def stable_timestep(state: ParityState, topology: SurfaceTopology, cfl: float = 0.85, **opts):
-    """v1_parity.py:153-158 on the GPU."""
-    with HipParitySolver(topology, state=state, cfl=cfl, **opts) as s:
-        return s.stable_timestep()
+    """v1_parity.py:153-158 on the GPU (the topology stays resident between calls
+    with the same arrays, as in advance_parity_step)."""
+    s = _contexts.get(("cfl", topology, float(cfl), opts),
+                      lambda: HipParitySolver(topology, cfl=cfl, **opts))
+    s.set_state(state)
+    return s.stable_timestep()
 
 
 class DeviceParityInputs(NamedTuple):
