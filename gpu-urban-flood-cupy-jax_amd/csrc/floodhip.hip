@@ -2046,13 +2046,16 @@ public:
         ensure_packed();
         std::vector<KernelTiming> out;
         // every launch carries start/stop events in its dispatch packet; the launches
-        // run back to back as in a step, and the events are read after the last one
+        // run back to back as in a step, and the events are read after the last one.
+        // The events take no system-scope fence (the stream synchronization before
+        // they are read orders them), so, as between the kernels of a step graph, the
+        // L2 is neither written back nor invalidated between the timed launches.
         struct Pending { std::string name; double bytes; hipEvent_t a, b; };
         std::vector<Pending> pending;
         auto timed = [&](const char* name, double bytes, auto&& fn) {
             hipEvent_t a, b;
-            FH_CHECK(hipEventCreate(&a));
-            FH_CHECK(hipEventCreate(&b));
+            FH_CHECK(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+            FH_CHECK(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
             ev_start = a;
             ev_stop = b;
             fn();
