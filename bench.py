@@ -241,7 +241,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=400, help="steps of the C oracle (all host threads)")
+    ap.add_argument("--cpu-steps", type=int, default=1000, help="steps of the C oracle (all host threads)")
     ap.add_argument("--numpy-steps", type=int, default=60, help="steps of the NumPy oracle (one process)")
     ap.add_argument("--wet-steps", type=int, default=2000)
     ap.add_argument("--no-wet", action="store_true", help="skip the wet-regime leg")

@@ -23,7 +23,8 @@ namespace fh {
 //   right == -1: ghost copies the left state (transmissive / unsupported codes)
 //   right == -2: reflective ghost (codes 0 and 2, or an inactive right cell)
 //   right == -3: outflow-only boundary (code 4): copy, inflow blocked
-// Edges whose left cell is inactive contribute exactly nothing and are dropped.
+// Edges whose left cell is inactive contribute exactly nothing and are dropped,
+// except code-3 boundary edges (kGhostDeadStage / kGhostDeadOther below).
 // ---------------------------------------------------------------------------
 constexpr int kGhostCopy = -1;
 constexpr int kGhostReflect = -2;
@@ -37,9 +38,18 @@ constexpr int kGhostStage = -4;
 // stage values) and any code other than 0-4: a copy ghost in the full step, a
 // wall in the local-inertial step
 constexpr int kGhostOther = -5;
+// A code-3 boundary edge whose cell is inactive is kept, unlike the other edges
+// of inactive cells: the V1 step computes its flux from a zero left state, so a
+// stage above the cell's bed flows in, is counted as boundary inflow and is then
+// removed with the inactive cell's water (v1_parity.py:170-214,
+// triangular_backend.py:1294-1327).  Without stage values the flux is zero.
+// Both kinds are walls in the local-inertial step.
+constexpr int kGhostDeadStage = -6;
+constexpr int kGhostDeadOther = -7;
 
-// cell -> edge table entry: (edge_position << 1) | is_right_side, -1 none,
-// -2 in slot 0 marks an inactive (or padding) cell.
+// cell -> edge table entry: (edge_position << 2) | (boundary << 1) | is_right_side, -1 none,
+// -2 in slot 0 marks an inactive (or padding) cell, whose slots 1 and 2 hold its
+// kept code-3 boundary edges, if any.
 constexpr int kCellInactive = -2;
 
 template <typename real>
@@ -436,7 +446,7 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         if (e >= end) continue;
         const int r = rgt[k];
         const real nx = nxs[k], ny = nys[k], len = lens[k];
-        const V sl = sls[k];
+        V sl = sls[k];
         real h_r, hu_r, hv_r, z_r;
         if (r >= 0) {
             const V sr = srs[k];
@@ -444,10 +454,14 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         } else if (r == kGhostReflect) {
             const real nm = sl.y * nx + sl.z * ny;
             h_r = sl.x; hu_r = sl.y - real(2) * nm * nx; hv_r = sl.z - real(2) * nm * ny; z_r = sl.w;
-        } else if (r == kGhostStage) {
+        } else if (r == kGhostStage || r == kGhostDeadStage) {
             h_r = stage_h[e]; hu_r = real(0); hv_r = real(0); z_r = sl.w;
         } else {
             h_r = sl.x; hu_r = sl.y; hv_r = sl.z; z_r = sl.w;
+        }
+        if (r <= kGhostDeadStage) {   // inactive cell: zero left state (and copy ghost)
+            sl.x = real(0); sl.y = real(0); sl.z = real(0);
+            if (r == kGhostDeadOther) { h_r = real(0); hu_r = real(0); hv_r = real(0); }
         }
         const EdgeFlux<real> f = edge_flux<real>(sl.x, sl.y, sl.z, sl.w, h_r, hu_r, hv_r, z_r, nx, ny, len,
                                                  r == kGhostOutflow, p);
@@ -456,7 +470,7 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         rec[e] = o;
         if (r >= 0) recR[e] = f.fvR;
         if (eval) {
-            const real cl = cell_cfl(p, sl, lc[lft[k]]);
+            const real cl = cell_cfl(p, sls[k], lc[lft[k]]);
             mn = cl < mn ? cl : mn;
             if (r >= 0) {
                 const real cr = cell_cfl(p, srs[k], lc[r]);
@@ -758,6 +772,17 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
         wt = wt + (s.x > eps ? dt_wet : zero);
         const int e0 = ent[0];
         if (e0 == kCellInactive) {
+            if constexpr (!LI) {
+                // code-3 boundary edges of the inactive cell (slots 1 and 2): their
+                // stage inflow counts as boundary volume, as for an active cell
+#pragma unroll
+                for (int k = 1; k < 3; ++k) {
+                    if (ent[k] < 0 || !(ent[k] & 2)) continue;
+                    const double outward = (double)(rr[k].x * dt);
+                    bin += fmax(-outward, 0.0);
+                    bout += fmax(outward, 0.0);
+                }
+            }
             s.x = zero; s.y = zero; s.z = zero;
             state[c] = s;
             wet[c] = wt;

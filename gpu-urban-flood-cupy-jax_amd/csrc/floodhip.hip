@@ -179,6 +179,7 @@ struct Member {
     // edge ids; stage = one value per caller edge (empty: no stage values)
     int64_t E_caller = 0;
     std::vector<int> st_pos, st_edge;
+    std::vector<char> st_dead;   // the code-3 edge's cell is inactive
     std::vector<float> stage;
     // drainage
     int64_t N = 0, I = 0, L = 0, S = 0, R = 0, Rcv = 0, T = 0;
@@ -447,7 +448,11 @@ public:
         std::vector<int> ghost_kind;
         for (int64_t e = 0; e < E_; ++e) {
             const int32_t l = el[e], r = er[e];
-            if (!act(l)) continue;
+            if (!act(l)) {
+                // kept: a stage can flow into the inactive cell (kGhostDeadStage)
+                if (r < 0 && code[e] == 3) { ghost.push_back(e); ghost_kind.push_back(fh::kGhostDeadOther); }
+                continue;
+            }
             const bool in = r >= 0;
             const bool ra = in && act(r);
             if (in && ra) { interior.push_back(e); continue; }
@@ -470,7 +475,7 @@ public:
         s.el.assign(s.E, 0); s.er.assign(s.E, 0);
         s.nx.assign(s.E, 0); s.ny.assign(s.E, 0); s.len.assign(s.E, 0);
         s.E_caller = E_;
-        s.st_pos.clear(); s.st_edge.clear(); s.stage.clear();
+        s.st_pos.clear(); s.st_edge.clear(); s.st_dead.clear(); s.stage.clear();
         for (int64_t i = 0; i < s.E; ++i) {
             const bool gh = i >= s.E_int;
             const int64_t e = gh ? ghost[i - s.E_int] : interior[i];
@@ -478,7 +483,9 @@ public:
             s.el[i] = el[e];
             s.er[i] = gh ? ghost_kind[i - s.E_int] : er[e];
             s.nx[i] = (real)nx[e]; s.ny[i] = (real)ny[e]; s.len[i] = (real)len[e];
-            if (gh && er[e] < 0 && code[e] == 3) { s.st_pos.push_back((int)i); s.st_edge.push_back((int)e); }
+            if (gh && er[e] < 0 && code[e] == 3) {
+                s.st_pos.push_back((int)i); s.st_edge.push_back((int)e); s.st_dead.push_back(act(el[e]) ? 0 : 1);
+            }
         }
         // cell -> edge table: left-side entries (ascending original edge), then right-side
         std::vector<int> lent(C_ * 3, -1), rent(C_ * 3, -1);
@@ -501,7 +508,11 @@ public:
             int k = 0;
             for (int i = 0; i < nl[c]; ++i) t[k++] = lent[c * 3 + i];
             for (int i = 0; i < nr[c]; ++i) t[k++] = rent[c * 3 + i];
-            if (!act(c)) t[0] = fh::kCellInactive;
+            if (!act(c)) {
+                // an inactive cell has only its kept code-3 boundary edges, in slots 1, 2
+                if (k > 2) throw ValueError{"an inactive cell with three code-3 boundary edges is not supported"};
+                t[2] = t[1]; t[1] = t[0]; t[0] = fh::kCellInactive;
+            }
             s.ce0[c] = t[0]; s.ce1[c] = t[1]; s.ce2[c] = t[2];
         }
         s.area.assign(C_, 0); s.n2.assign(C_, 0); s.n1.assign(C_, 0); s.lc.assign(C_, 0); s.bed.assign(C_, 0);
@@ -608,8 +619,10 @@ public:
         if (n == 0 || !stage) s.stage.clear();
         else if (n == 1) s.stage.assign((size_t)s.E_caller, stage[0]);
         else s.stage.assign(stage, stage + n);
-        const int kind = s.stage.empty() ? fh::kGhostOther : fh::kGhostStage;
-        for (int i : s.st_pos) s.er[(size_t)i] = kind;
+        const bool staged = !s.stage.empty();
+        for (size_t k = 0; k < s.st_pos.size(); ++k)
+            s.er[(size_t)s.st_pos[k]] = s.st_dead[k] ? (staged ? fh::kGhostDeadStage : fh::kGhostDeadOther)
+                                                     : (staged ? fh::kGhostStage : fh::kGhostOther);
     }
 
     void set_forcing(int64_t nr, const float* rs, const float* re, const float* rr, const float* m,

@@ -63,6 +63,44 @@ def test_stage_steps_match_oracle(precision):
     assert rdiag["boundary_inflow"] > 0 and rdiag["boundary_outflow"] > 0
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_stage_edges_of_inactive_cells_match_oracle(precision):
+    """Stage edges whose cell is inactive: the stage flows into the inactive cell,
+    counts as boundary inflow and is removed with the cell's water; the stage can
+    be set and removed after upload."""
+    inp, stage = _stage_scenario(seed=5)
+    s = inp.surface
+    st_edges = np.flatnonzero(s.boundary_code == 3)
+    above = st_edges[stage[st_edges] > s.bed[s.edge_left[st_edges]]]
+    active = s.active.copy()
+    active[s.edge_left[above[::2]]] = 0
+    inp = inp._replace(surface=s._replace(active=active))
+    dtype = np.float32 if precision == 32 else np.float64
+    io = to_oracle(inp, dtype)
+    dead = np.flatnonzero((io[0].edge_right < 0) & (io[0].active[io[0].edge_left] < 0.5) & (io[0].boundary_code == 3))
+    assert dead.size >= 3
+    st = io[4]
+    tol_h = 1e-5 if precision == 32 else 1e-10
+    with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
+                         precision=precision) as sol:
+        for i in range(16):
+            staged = i >= 4 and i < 12       # no stage, stage values, removed again
+            if i == 4:
+                sol.set_boundary_stage(stage)
+            if i == 12:
+                sol.set_boundary_stage(None)
+            sm = stage if staged else None
+            st, rdiag = oracle_step(io, st, 0.4, rain=2e-5, dtype=dtype, boundary_stage_m=sm)
+            diag = sol.step(0.4, 2e-5)
+            got = sol.get_state_f64()[0] if precision == 64 else sol.get_state().h
+            assert np.max(np.abs(got - st.h)) <= tol_h, i
+            for k in ("boundary_inflow", "boundary_outflow", "rainfall_input"):
+                r = rdiag[k]
+                assert abs(getattr(diag, k) - r) <= 1e-4 * max(abs(r), 1e-3), (i, k, getattr(diag, k), r)
+            if staged:
+                assert rdiag["boundary_inflow"] > 0
+
+
 def test_stage_policy_advance_and_scalar_stage():
     inp = basin(n=24, extent=480.0, code="stage", bed=lambda c: 0.002 * c[:, 0],
                 h0=lambda c: np.full(c.shape[0], 0.3))

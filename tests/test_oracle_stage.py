@@ -109,3 +109,39 @@ def test_without_stage_values_code_3_copies_the_left_state():
     touched = np.unique(io[0].edge_left[b[:5]])
     others = np.setdiff1d(np.arange(st.h.size), touched)
     assert np.all(st.h[touched] > 1.0) and np.max(np.abs(st.h[others] - 1.0)) < 1e-12
+
+
+def test_stage_at_an_inactive_cell_flows_in_and_is_removed():
+    """The V1 step masks an inactive cell's state to zero but still computes the flux
+    of its boundary edges (every edge that is not an active interior edge is a
+    boundary edge, v1_parity.py:170-214): a stage above the cell's bed gives the
+    textbook HLL inflow of (0, 0, 0) against (stage - z, 0, 0), counted as boundary
+    inflow, and the cell's new depth is then zeroed.  Without stage values such an
+    edge carries nothing."""
+    z0, stage, dt = 0.5, 1.3, 0.05
+    inp = _stage_basin(bed=lambda c: np.full(c.shape[0], z0), h0=lambda c: np.full(c.shape[0], 0.2))
+    s = inp.surface
+    b = np.flatnonzero(s.edge_right < 0)
+    dead_cells = np.unique(s.edge_left[b[::7]])
+    active = s.active.copy()
+    active[dead_cells] = 0
+    inp = inp._replace(surface=s._replace(active=active))
+    io = list(to_oracle(inp, np.float64))
+    s = io[0]
+    st, led = _run(io, 1, dt, stage)
+    dead_edges = np.flatnonzero((s.edge_right < 0) & (s.active[s.edge_left] < 0.5))
+    assert dead_edges.size >= 2
+    inflow_dead = sum(-_hll_rotated(0.0, 0.0, 0.0, stage - z0, 0.0, 0.0,
+                                    (float(s.edge_nx[e]), float(s.edge_ny[e])))[0]
+                      * float(s.edge_length[e]) * dt for e in dead_edges)
+    assert inflow_dead > 0
+    # the active cells' boundary edges: the same stage against the depth at rest
+    live = np.flatnonzero((s.edge_right < 0) & (s.active[s.edge_left] > 0.5))
+    h0 = float(io[4].h[0])      # float32 0.2
+    inflow_live = sum(-_hll_rotated(h0, 0.0, 0.0, stage - z0, 0.0, 0.0,
+                                    (float(s.edge_nx[e]), float(s.edge_ny[e])))[0]
+                      * float(s.edge_length[e]) * dt for e in live)
+    assert abs(led["boundary_inflow"] - (inflow_dead + inflow_live)) <= 1e-9 * led["boundary_inflow"]
+    assert np.all(st.h[dead_cells] == 0)
+    _, led0 = _run(io, 1, dt, None)
+    assert led0["boundary_inflow"] < 1e-12
