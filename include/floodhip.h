@@ -158,7 +158,9 @@ int fh_set_forcing(fh_ctx* ctx, int64_t n_intervals, const float* rain_start_s,
  * fh_upload_surface: one per edge in the caller's order; 0 or NULL: none, and
  * code 3 copies the left state as v1_parity.py's step does.  The local-inertial
  * step treats code 3 as a wall either way.  A stage edge whose own (left) cell
- * is inactive is dropped with every other edge of an inactive cell. */
+ * is inactive lets the stage flow in against a zero state; that inflow counts as
+ * boundary volume and is removed with the inactive cell's water, as in the V1
+ * step (v1_parity.py:170-214). */
 int fh_set_boundary_stage(fh_ctx* ctx, int64_t n_values, const float* stage_m);
 
 /* ParityState (v1_parity.py:80-87).  NULL leaves a field unchanged (set) or unread (get). */

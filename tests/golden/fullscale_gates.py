@@ -31,7 +31,9 @@ host cores) and a GPU half (the HIP solver), joined by a comparison on the CPU:
 
    resync-300 restarts the GPU from the oracle's float32 state before every step
    and compares one step at a time (the one-step error); trace-300 follows both
-   FP64 trajectories step by step at the outfall nodes.
+   FP64 trajectories step by step (depths, node volumes, outfall nodes).  With
+   --surface-mode local_inertial the 300 s procedures run the local-inertial
+   step, against the NumPy oracle (the C restatement covers the full step).
 """
 
 from __future__ import annotations
@@ -181,31 +183,54 @@ def _dry_inputs():
     return synth.make_production_scale()
 
 
-def oracle_300(out_dir: Path, threads: int, t_end=300.0):
+def _tag(dtype_tag: str, mode: str) -> str:
+    return dtype_tag if mode == "full_swe" else f"li_{dtype_tag}"
+
+
+class _Oracle:
+    """The oracle's CFL-policy advance: the C restatement for the full step, the
+    NumPy oracle for the local-inertial step (which the C restatement does not
+    cover)."""
+
+    def __init__(self, io, dtype, mode):
+        self.io, self.dtype, self.mode = io, dtype, mode
+        if mode == "full_swe":
+            from oracle import v1_parity_c as oc
+            s, c, p, r, _, f = io
+            self.co = oc.COracle(s, c, p, r, f, dtype=dtype)
+
+    def advance(self, st, clock, ledger, t_end, max_steps):
+        if self.mode == "full_swe":
+            return self.co.advance_to_time(st, clock, ledger, t_end, DURATION, max_steps=max_steps)
+        s, c, p, r, _, f = self.io
+        return orc.advance_to_time(st, clock, ledger, t_end, DURATION, s, c, p, r, f, max_steps=max_steps,
+                                   dtype=self.dtype, surface_mode=self.mode)
+
+
+def oracle_300(out_dir: Path, threads: int, t_end=300.0, mode="full_swe"):
     from oracle import v1_parity_c as oc
     out_dir.mkdir(parents=True, exist_ok=True)
     if threads:
         oc.set_threads(threads)
     inp = _dry_inputs()
     for dtype, tag in ((np.float32, "f32"), (np.float64, "f64")):
-        s, c, p, r, st, f = to_oracle(inp, dtype)
-        co = oc.COracle(s, c, p, r, f, dtype=dtype)
+        io = to_oracle(inp, dtype)
         t0 = time.time()
-        st, clock, led = co.advance_to_time(st, orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger(), t_end, DURATION,
-                                            max_steps=100_000)
-        print(f"oracle {tag}: {clock.global_step} steps in {time.time() - t0:.1f} s", flush=True)
-        np.savez_compressed(out_dir / f"oracle_300_{tag}.npz", h=st.h, node_volume=st.node_volume,
+        st, clock, led = _Oracle(io, dtype, mode).advance(io[4], orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger(),
+                                                          t_end, 100_000)
+        print(f"oracle {_tag(tag, mode)}: {clock.global_step} steps in {time.time() - t0:.1f} s", flush=True)
+        np.savez_compressed(out_dir / f"oracle_300_{_tag(tag, mode)}.npz", h=st.h, node_volume=st.node_volume,
                             link_flow=st.link_flow, steps=clock.global_step, time_s=clock.time_s,
                             ledger=json.dumps(led))
 
 
-def gpu_300(out: Path, t_end=300.0):
+def gpu_300(out: Path, t_end=300.0, mode="full_swe"):
     from floodhip.solver import HipParitySolver
     inp = _dry_inputs()
     arrays = {}
     for prec in (32, 64):
         with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
-                             precision=prec) as s:
+                             precision=prec, surface_mode=mode) as s:
             c, led = s.advance_to(t_end, DURATION, max_steps=100_000)
             st = s.get_state()
             h = s.get_state_f64()[0] if prec == 64 else st.h
@@ -215,12 +240,13 @@ def gpu_300(out: Path, t_end=300.0):
     np.savez_compressed(out, **arrays)
 
 
-def compare_300(oracle_dir: Path, gpu_npz: Path, out_json: Path):
+def compare_300(oracle_dir: Path, gpu_npz: Path, out_json: Path, mode="full_swe"):
     inp = _dry_inputs()
     g = np.load(gpu_npz, allow_pickle=False)
-    out = {"workload": "981,880-triangle coupled production workload, dry start, CFL policy to t = 300 s"}
+    out = {"workload": "981,880-triangle coupled production workload, dry start, CFL policy to t = 300 s",
+           "surface_mode": mode}
     for prec, tag in ((32, "f32"), (64, "f64")):
-        o = np.load(oracle_dir / f"oracle_300_{tag}.npz", allow_pickle=False)
+        o = np.load(oracle_dir / f"oracle_300_{_tag(tag, mode)}.npz", allow_pickle=False)
         vg = g[f"node_volume_{prec}"].astype(np.float64)
         vo = o["node_volume"].astype(np.float64)
         rel = np.abs(vg - vo) / np.maximum(np.abs(vo), 1.0)
@@ -247,25 +273,27 @@ def compare_300(oracle_dir: Path, gpu_npz: Path, out_json: Path):
                       for k, v in out.items() if k in ("f32", "f64")}))
 
 
-def resync_300(out_json: Path, t_end=300.0):
+def resync_300(out_json: Path, t_end=300.0, mode="full_swe"):
     """Float32 one-step error along the oracle's trajectory: before every step the
     GPU takes the oracle's state (float32, exact), then both take the same step
     (the oracle's dt and rain rate); the step results are compared node by node.
     A kernel that computes the oracle's step differs only by transcendental
     rounding in every step, however far the two free-running trajectories drift."""
     from floodhip.solver import HipParitySolver
-    from oracle import v1_parity_c as oc
     inp = _dry_inputs()
     io = to_oracle(inp)
     s_, c_, p_, r_, st, f_ = io
-    co = oc.COracle(s_, c_, p_, r_, f_)
     outfall = np.asarray(inp.pipe.outfall_area) > 0
     rows = []
+    oracle = _Oracle(io, np.float32, mode)
     clock, ledger = orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger()
-    with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state) as s:
+    with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
+                         surface_mode=mode) as s:
         while clock.time_s < t_end:
             before = st
-            st, c1, ledger = co.advance_to_time(st, clock, ledger, t_end, DURATION, max_steps=1)
+            st, c1, ledger = oracle.advance(before, clock, ledger, t_end, 1)
+            if clock.global_step % 20 == 0:
+                print(f"resync step {clock.global_step}, t = {clock.time_s:.2f} s", flush=True)
             dt = c1.time_s - clock.time_s
             rate = orc.rain_rate(f_, clock.time_s)
             s.set_state(before)
@@ -280,39 +308,49 @@ def resync_300(out_json: Path, t_end=300.0):
     worst = {k: max(r[k] for r in rows) for k in ("dh_max", "node_rel_max", "outfall_node_rel_max", "link_rel_max")}
     out = {"workload": "981,880-triangle coupled production workload, dry start, oracle trajectory to t = 300 s; "
                        "the GPU restarts every step from the oracle's float32 state",
-           "steps": len(rows), "worst_one_step_error": worst, "per_step": rows}
+           "surface_mode": mode, "steps": len(rows), "worst_one_step_error": worst, "per_step": rows}
     out_json.write_text(json.dumps(out, indent=1) + "\n")
     print(json.dumps({"steps": len(rows), **worst}), flush=True)
 
 
-def trace_300(out_npz: Path, t_end=300.0):
-    """FP64 on both sides, free-running under the CFL policy: the node volumes of
-    every outfall node after every step, and the largest relative node-volume
-    difference over all nodes, to show where and how fast the trajectories part."""
+def trace_300(out_npz: Path, t_end=300.0, mode="full_swe"):
+    """FP64 on both sides, free-running under the CFL policy: after every step the
+    largest depth difference, the largest relative node-volume difference and the
+    node volumes of every outfall node, with both dts, to show where and how fast
+    the trajectories part."""
     from floodhip.solver import HipParitySolver
-    from oracle import v1_parity_c as oc
     inp = _dry_inputs()
     io = to_oracle(inp, np.float64)
-    s_, c_, p_, r_, st, f_ = io
-    co = oc.COracle(s_, c_, p_, r_, f_, dtype=np.float64)
+    st = io[4]
+    oracle = _Oracle(io, np.float64, mode)
     outfall = np.flatnonzero(np.asarray(inp.pipe.outfall_area) > 0)
     clock, ledger = orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger()
-    vo, vg, relmax, dts = [], [], [], []
+    vo, vg, relmax, dhmax, dts = [], [], [], [], []
     with HipParitySolver(inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing, inp.initial_state,
-                         precision=64) as s:
+                         precision=64, surface_mode=mode) as s:
         while clock.time_s < t_end:
-            st, c1, ledger = co.advance_to_time(st, clock, ledger, t_end, DURATION, max_steps=1)
+            st, c1, ledger = oracle.advance(st, clock, ledger, t_end, 1)
             gc, _ = s.advance_to(t_end, DURATION, max_steps=1, raise_on_budget=False)
             v, _ = s.get_drainage_f64()
+            h = s.get_state_f64()[0]
             vo.append(np.asarray(st.node_volume)[outfall])
             vg.append(v[outfall])
             relmax.append(float(np.max(np.abs(v - st.node_volume) / np.maximum(np.abs(st.node_volume), 1.0))))
+            dhmax.append(float(np.max(np.abs(h - st.h))))
             dts.append((c1.time_s - clock.time_s, gc["last_dt_s"]))
+            if clock.global_step % 20 == 0:
+                print(f"trace step {clock.global_step}: max |dh| {dhmax[-1]:.3e} m, node rel {relmax[-1]:.3e}",
+                      flush=True)
             clock = c1
     np.savez_compressed(out_npz, outfall_nodes=outfall, oracle=np.stack(vo), gpu=np.stack(vg),
-                        node_rel_max=np.asarray(relmax), dt=np.asarray(dts))
-    first = {thr: next((i for i, x in enumerate(relmax) if x > thr), None) for thr in (1e-13, 1e-10, 1e-7, 1e-4, 1e-2)}
-    print(json.dumps({"steps": len(relmax), "first_step_above": first, "final_rel_max": relmax[-1]}), flush=True)
+                        node_rel_max=np.asarray(relmax), depth_max_abs=np.asarray(dhmax), dt=np.asarray(dts))
+    thr = (1e-13, 1e-10, 1e-7, 1e-4, 1e-2)
+    first = {t: next((i for i, x in enumerate(relmax) if x > t), None) for t in thr}
+    first_h = {t: next((i for i, x in enumerate(dhmax) if x > t), None) for t in thr}
+    dt_diff = next((i for i, (a, b) in enumerate(dts) if a != b), None)
+    print(json.dumps({"steps": len(relmax), "node_first_step_above": first, "depth_first_step_above": first_h,
+                      "final_node_rel_max": relmax[-1], "final_depth_max_abs": dhmax[-1],
+                      "dt_first_differs_at_step": dt_diff}), flush=True)
 
 
 def main():
@@ -322,6 +360,8 @@ def main():
     ap.add_argument("paths", nargs="+")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--duration", type=float, default=DURATION)
+    ap.add_argument("--surface-mode", default="full_swe", choices=["full_swe", "local_inertial"],
+                    help="surface operator of the 300 s procedures")
     a = ap.parse_args()
     P = [Path(x) for x in a.paths]
     if a.mode == "oracle-event":
@@ -331,15 +371,15 @@ def main():
     elif a.mode == "gates":
         gates(P[0], P[1], P[2])
     elif a.mode == "oracle-300":
-        oracle_300(P[0], a.threads)
+        oracle_300(P[0], a.threads, mode=a.surface_mode)
     elif a.mode == "gpu-300":
-        gpu_300(P[0])
+        gpu_300(P[0], mode=a.surface_mode)
     elif a.mode == "compare-300":
-        compare_300(P[0], P[1], P[2])
+        compare_300(P[0], P[1], P[2], mode=a.surface_mode)
     elif a.mode == "resync-300":
-        resync_300(P[0])
+        resync_300(P[0], mode=a.surface_mode)
     else:
-        trace_300(P[0])
+        trace_300(P[0], mode=a.surface_mode)
 
 
 if __name__ == "__main__":
