@@ -19,6 +19,10 @@
  *   return_node_overflow         v1_parity.py:371-404
  *   external_inflow, recharge    v1_parity.py:407-427
  *   oracle_step                  v1_parity.py:430-469 (V1 process order)
+ *   li_stable_timestep,          the local-inertial surface mode (oracle/local_inertial_np.py):
+ *   li_surface_step              gurugram_flood/kernels/triangular_backend.py:1181-1291,
+ *                                local_inertial.py:20-48, network_kernels.py:45-82,
+ *                                triangular_backend.py:1767-1772, equations.py:181-207
  *
  * Arithmetic: every operation in REAL (float or double; the file is compiled
  * once for each), rounded per operation (-ffp-contract=off, no fast-math), in
@@ -106,6 +110,8 @@ typedef struct {
     /* forcing fields */
     const R *rainm, *f0, *fc, *hk;
     const int32_t* rcv;
+    /* surface operator: 0 the full shallow-water step, 1 the local-inertial step */
+    int32_t li;
 } Topo;
 
 /* Scratch buffers (caller-allocated, reused every step). */
@@ -115,6 +121,7 @@ typedef struct {
     R *cell_tmp, *node_tmp, *node_tmp2, *head, *inlet_a, *inlet_b, *link_a, *link_b, *entry_a;
     uint8_t* link_pos;
     double *sum_a, *sum_b;                /* max(E, C) each: operands of the FP64 ledger sums */
+    R *cell_b, *cell_c;                   /* per cell (local-inertial step) */
 } Work;
 
 static void chunk_sum(const double* v, int64_t n, double* out) {
@@ -184,8 +191,12 @@ static void hydrostatic_flux_pair(R h_l, R hu_l, R hv_l, R z_l, R h_r, R hu_r, R
     fr[0] = f[0] + zero; fr[1] = f[1] + pc_r * nx; fr[2] = f[2] + pc_r * ny;
 }
 
-/* v1_parity.py:153-158: min over cells of cfl Lc / (|u| + sqrt(g h)), non-finite skipped */
+static R li_stable_timestep(const Topo* t, const R* h, double cfl);
+
+/* v1_parity.py:153-158: min over cells of cfl Lc / (|u| + sqrt(g h)), non-finite skipped
+ * (the local-inertial mode: its gravity-wave timestep) */
 R API(oracle_stable_timestep)(const Topo* t, const R* h, const R* hu, const R* hv, double cfl) {
+    if (t->li) return li_stable_timestep(t, h, cfl);
     const R eps = (R)MIN_DEPTH, c = (R)cfl;
     R best = (R)INFINITY;
 #pragma omp parallel
@@ -205,6 +216,143 @@ R API(oracle_stable_timestep)(const Topo* t, const R* h, const R* hu, const R* h
         if (mn < best) best = mn;
     }
     return best;
+}
+
+/* equations.py:181-207: min over cells of cfl Lc / max(sqrt(g max(h, min depth)), 1e-12) */
+static R li_stable_timestep(const Topo* t, const R* h, double cfl) {
+    const R c = (R)cfl;
+    R best = (R)INFINITY;
+#pragma omp parallel
+    {
+        R mn = (R)INFINITY;
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < t->C; ++i) {
+            const R wave = SQRT((R)GRAVITY * vmax(h[i], (R)MIN_DEPTH));
+            R local = c * t->lc[i] / vmax(wave, (R)1.0e-12);
+            if (!isfinite(local)) local = (R)INFINITY;
+            if (local < mn) mn = local;
+        }
+#pragma omp critical
+        if (mn < best) best = mn;
+    }
+    return best;
+}
+
+/* local_inertial_np.surface_step (triangular_backend.py:1181-1291).  Every
+ * scatter-add of the NumPy form is a per-cell gather in ascending edge order:
+ * the donor sums merge the cell's left-side and right-side edge lists. */
+static void li_surface_step(const Topo* t, Work* w, R* h, R* hu, R* hv, R dt, double* diag) {
+    const R zero = (R)0, half = (R)0.5, g = (R)GRAVITY, eps = (R)MIN_DEPTH;
+    const R gdt = (R)(GRAVITY * (double)dt);
+    R *q = w->fl0, *T = w->fl1, *outq = w->fl2, *qx = w->fr0, *qy = w->fr1;
+    R *factor = w->cell_tmp, *idelta = w->cell_b, *cfac = w->cell_c;
+    /* edge discharge (local_inertial.py:20-48) and the requested transfers */
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < t->E; ++e) {
+        const int32_t left = t->edge_left[e], right = t->edge_right[e];
+        const int interior = right >= 0;
+        const int32_t sr = interior ? right : left;
+        const int la = t->active[left] != 0, ra = t->active[sr] != 0;
+        const int ai = interior && la && ra;
+        const int32_t code = (interior && la && !ra) ? 0 : t->code[e];
+        const R h_l = la ? h[left] : zero, hu_l = la ? hu[left] : zero, hv_l = la ? hv[left] : zero;
+        const R h_r = ra ? h[sr] : zero, hu_r = ra ? hu[sr] : zero, hv_r = ra ? hv[sr] : zero;
+        const R bed_l = t->bed[left], bed_r = t->bed[sr];
+        const R eta_l = bed_l + h_l;
+        const R eta_r = ai ? bed_r + h_r : bed_l;
+        const R bed_star = ai ? vmax(bed_l, bed_r) : bed_l;
+        const R h_flow = vmax(vmax(eta_l, eta_r) - bed_star, zero);
+        const R L = t->len[e], nx = t->nx[e], ny = t->ny[e];
+        const R distance = vmax(half * (t->lc[left] + t->lc[sr]), (R)1.0e-6);
+        const R slope = (eta_r - eta_l) / distance;
+        const R nm_l = hu_l * nx + hv_l * ny, nm_r = hu_r * nx + hv_r * ny;
+        const R q_old = (ai ? half * (nm_l + nm_r) : nm_l) * L;
+        const R n_face = ai ? half * (t->manning[left] + t->manning[sr]) : t->manning[left];
+        const int bedge = !interior && la && (code == 4 || code == 1);
+        const R area = vmax(h_flow * L, (R)1.0e-8);
+        const R radius = vmax(h_flow, (R)1.0e-6);
+        const R numerator = q_old - g * area * dt * slope;
+        const R denominator = (R)1 + (gdt * (n_face * n_face) * FABS(q_old) / (POW(radius, (R)(4.0 / 3.0)) * area));
+        R qq = numerator / denominator;
+        qq = (ai || bedge) ? qq : zero;
+        if (bedge) qq = vmax(qq, zero);
+        q[e] = qq;
+        T[e] = ai ? qq * dt : zero;
+        outq[e] = bedge ? vmax(qq, zero) * dt : zero;
+        qx[e] = qq * nx;
+        qy[e] = qq * ny;
+    }
+    /* donor factors (network_kernels.py:45-67): the cell's |request| over the
+     * edges it donates to, in ascending edge order */
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < t->C; ++c) {
+        R sum = zero;
+        int64_t a = t->lptr[c], b = t->rptr[c];
+        const int64_t a1 = t->lptr[c + 1], b1 = t->rptr[c + 1];
+        while (a < a1 || b < b1) {
+            const int takel = b >= b1 || (a < a1 && t->lidx[a] < t->ridx[b]);
+            const int32_t e = takel ? t->lidx[a++] : t->ridx[b++];
+            const R r = T[e];
+            if (takel ? r >= zero : r < zero) sum = sum + FABS(r);
+        }
+        const R volume = vmax(h[c], zero) * t->area[c];
+        factor[c] = vmin((R)1, volume / vmax(sum, (R)1.0e-12));
+    }
+    /* limited signed transfers */
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < t->E; ++e) {
+        const R r = T[e];
+        const int32_t left = t->edge_left[e], right = t->edge_right[e];
+        const int32_t donor = r >= zero ? left : (right >= 0 ? right : left);
+        T[e] = (r >= zero ? (R)1 : (R)-1) * (FABS(r) * factor[donor]);
+    }
+    /* continuity (network_kernels.py:70-82: all 'from' entries, then all 'to'
+     * entries, one running sum) and the capture factor of the boundary outflow
+     * (triangular_backend.py:1767-1772) */
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < t->C; ++c) {
+        R acc = zero, by = zero;
+        for (int64_t k = t->lptr[c]; k < t->lptr[c + 1]; ++k) acc = acc + (-T[t->lidx[k]]);
+        for (int64_t k = t->rptr[c]; k < t->rptr[c + 1]; ++k) acc = acc + T[t->ridx[k]];
+        for (int64_t k = t->lptr[c]; k < t->lptr[c + 1]; ++k) by = by + vmax(outq[t->lidx[k]], zero);
+        idelta[c] = acc;
+        const R volume_after = vmax(vmax(h[c], zero) * t->area[c] + acc, zero);
+        cfac[c] = vmin((R)1, volume_after / vmax(by, (R)1.0e-12));
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < t->E; ++e) {
+        outq[e] = vmax(outq[e], zero) * cfac[t->edge_left[e]];
+        w->bpart[e] = (double)outq[e];
+    }
+    /* depth and the momentum rebuilt from the edge discharges */
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < t->C; ++c) {
+        R out = zero, sx = zero, sy = zero, rx = zero, ry = zero;
+        for (int64_t k = t->lptr[c]; k < t->lptr[c + 1]; ++k) {
+            const int32_t e = t->lidx[k];
+            out = out + outq[e];
+            sx = sx + qx[e];
+            sy = sy + qy[e];
+        }
+        for (int64_t k = t->rptr[c]; k < t->rptr[c + 1]; ++k) {
+            const int32_t e = t->ridx[k];
+            rx = rx + qx[e];
+            ry = ry + qy[e];
+        }
+        const R A = t->area[c];
+        const R delta = idelta[c] - out;
+        const R hh = t->active[c] ? vmax(h[c] + delta / A, zero) : zero;
+        const R perimeter = (R)2 * A / vmax(t->lc[c], (R)1.0e-12);
+        const int wet = hh > eps;
+        h[c] = hh;
+        hu[c] = wet ? (sx + rx) / vmax(perimeter, (R)1.0e-12) : zero;
+        hv[c] = wet ? (sy + ry) / vmax(perimeter, (R)1.0e-12) : zero;
+    }
+    double bout = 0.0;
+    chunk_sum(w->bpart, t->E, &bout);
+    diag[0] = 0.0 - bout;
+    diag[1] = 0.0;
+    diag[2] = bout;
 }
 
 /* v1_parity.py:161-214 (diag: boundary net, in, out) */
@@ -607,7 +755,8 @@ int API(oracle_step)(const Topo* t, Work* w, R* h, R* hu, R* hv, R* wet_time, R*
                      double slot_wave_speed, double* diag) {
     const R dt = (R)dt_in;
     double s3[3];
-    surface_flux_step(t, w, h, hu, hv, dt, s3);
+    if (t->li) li_surface_step(t, w, h, hu, hv, dt, s3);
+    else surface_flux_step(t, w, h, hu, hv, dt, s3);
     double rain, inf;
     sources_and_clamp(t, w, h, hu, hv, wet_time, dt, rain_rate, max_speed, &rain, &inf);
     const double ext = external_inflow(t, w, h, ext_q, dt);

@@ -4,7 +4,8 @@ TEST INFRASTRUCTURE ONLY, like oracle/v1_parity_np.py, whose functions the C
 file restates operation by operation (the reference lines are cited there):
 only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline``
 leg use it, as the checker or the timed CPU baseline.  It covers the full
-shallow-water step (the local-inertial surface mode is in NumPy only).
+shallow-water step and the local-inertial surface mode
+(``surface_mode="local_inertial"``, restating oracle/local_inertial_np.py).
 
 ``COracle`` holds one scenario's topology and forcing fields and steps a state
 in place; ``advance_to_time`` is the NumPy oracle's driver (parity_runner.py:
@@ -58,7 +59,7 @@ _TOPO_FIELDS = [(n, C.c_int64) for n in ("C", "E", "I", "N", "L", "S", "Rr", "X"
     (n, _P) for n in (
         "lfrom", "lto", "geom", "llen", "lman", "lcap", "linv", "D", "minor", "rw", "rh", "oarea", "ocoef",
         "otw", "ocap", "rcell", "rS", "rcc", "rrc", "rstop", "rainm", "f0", "fc", "hk", "rcv")
-]
+] + [("li", C.c_int32)]
 
 
 class _Topo(C.Structure):
@@ -68,7 +69,7 @@ class _Topo(C.Structure):
 class _Work(C.Structure):
     _fields_ = [(n, _P) for n in ("fl0", "fl1", "fl2", "fr0", "fr1", "fr2", "bpart", "cell_tmp", "node_tmp",
                                   "node_tmp2", "head", "inlet_a", "inlet_b", "link_a", "link_b", "entry_a",
-                                  "link_pos", "sum_a", "sum_b")]
+                                  "link_pos", "sum_a", "sum_b", "cell_b", "cell_c")]
 
 
 _LIBS = {}
@@ -115,7 +116,9 @@ class COracle:
     """One scenario (oracle NamedTuples in ``dtype``), stepped in place on the CPU."""
 
     def __init__(self, surface, coupling, pipe, recharge, forcing, dtype=np.float32, boundary_stage_m=None,
-                 capture_q=0.62 * 0.04, slot_wave_speed=30.0, max_speed=5.0):
+                 capture_q=0.62 * 0.04, slot_wave_speed=30.0, max_speed=5.0, surface_mode="full_swe"):
+        if surface_mode not in ("full_swe", "local_inertial"):
+            raise ValueError(f"unknown surface_mode {surface_mode!r}")
         self.dtype = np.dtype(dtype)
         self._step, self._cfl, _ = _load(dtype)
         self.capture_q, self.slot_wave_speed, self.max_speed = capture_q, slot_wave_speed, max_speed
@@ -171,6 +174,7 @@ class COracle:
                           ("hk", f.horton_k_per_s)):
             setattr(t, name, arr(src, R))
         t.rcell, t.rcv = arr(r.source_cell, I32), arr(f.external_receiver_cell, I32)
+        t.li = 1 if surface_mode == "local_inertial" else 0
         self.topo = t
         w = _Work()
         E, N, L = t.E, t.N, t.L
@@ -186,6 +190,7 @@ class COracle:
         w.link_pos = arr(np.zeros(max(L, N), np.uint8), np.uint8)
         w.sum_a = arr(np.zeros(max(E, Cn)), np.float64)
         w.sum_b = arr(np.zeros(max(E, Cn)), np.float64)
+        w.cell_b, w.cell_c = arr(np.zeros(Cn, R), R), arr(np.zeros(Cn, R), R)
         self.work = w
 
     def _state_arrays(self, state):

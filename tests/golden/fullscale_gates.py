@@ -80,14 +80,14 @@ def _report(inp, state, ledger, depth, steps):
             "mass_residual_ratio": (lhs - rhs) / max(abs(lhs), 1.0), "step_count": int(steps)}
 
 
-def oracle_event(out_dir: Path, threads: int, duration=DURATION):
+def oracle_event(out_dir: Path, threads: int, duration=DURATION, mode="full_swe"):
     from oracle import v1_parity_c as oc
     out_dir.mkdir(parents=True, exist_ok=True)
     if threads:
         oc.set_threads(threads)
     inp = event_inputs()
     s, c, p, r, st, f = to_oracle(inp)
-    co = oc.COracle(s, c, p, r, f)
+    co = oc.COracle(s, c, p, r, f, surface_mode=mode)
     targets = [float(np.float32(1.0e-6))] + [float(x) for x in output_targets(duration)[1:]]
     ckpt = out_dir / "checkpoint.npz"
     clock, ledger, snaps, k, steps_in = orc.OracleClock(0.0, 1.0, 0), orc.zero_ledger(), [], 0, 0
@@ -137,11 +137,12 @@ def oracle_event(out_dir: Path, threads: int, duration=DURATION):
     print(json.dumps(report), flush=True)
 
 
-def gpu_event(out: Path, duration=DURATION):
+def gpu_event(out: Path, duration=DURATION, mode="full_swe"):
     from floodhip.runner import run_hip_v1_parity
     inp = event_inputs()
     t0 = time.time()
-    res = run_hip_v1_parity(inp, duration_s=duration, max_steps_per_hour=BUDGET, keep_final_state=False)
+    res = run_hip_v1_parity(inp, duration_s=duration, max_steps_per_hour=BUDGET, keep_final_state=False,
+                            surface_mode=mode)
     el = time.time() - t0
     rep = res["report"]
     rep["wall_s"] = el
@@ -156,7 +157,7 @@ def _load_record(path):
     return rec
 
 
-def gates(oracle_npz: Path, gpu_npz: Path, out_json: Path):
+def gates(oracle_npz: Path, gpu_npz: Path, out_json: Path, mode="full_swe"):
     from floodhip import gates as g
     ref, cand = _load_record(oracle_npz), _load_record(gpu_npz)
     inp = event_inputs()
@@ -166,7 +167,7 @@ def gates(oracle_npz: Path, gpu_npz: Path, out_json: Path):
                     - ref["drain_node_volume_snapshots_m3"][-1]) /
              np.maximum(np.abs(ref["drain_node_volume_snapshots_m3"][-1]), 1.0))
     summary = {"workload": "981,880-triangle coupled production workload, 10 cm wet start, one hour, float32; "
-                           "reference: the CPU oracle (C restatement, all host cores)",
+                           "reference: the CPU oracle (C restatement, all host cores)", "surface_mode": mode,
                "passed": bool(res["passed"]), "gates": res["gates"], "summary": res["summary"],
                "steps": {"gpu": int(cand["report"]["step_count"]), "oracle": int(ref["report"]["step_count"])},
                "last_snapshot": {"max_abs_dh_m": float(np.max(np.abs(d))),
@@ -361,15 +362,15 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--duration", type=float, default=DURATION)
     ap.add_argument("--surface-mode", default="full_swe", choices=["full_swe", "local_inertial"],
-                    help="surface operator of the 300 s procedures")
+                    help="surface operator of the runs (the gates mode compares what it is given)")
     a = ap.parse_args()
     P = [Path(x) for x in a.paths]
     if a.mode == "oracle-event":
-        oracle_event(P[0], a.threads, a.duration)
+        oracle_event(P[0], a.threads, a.duration, mode=a.surface_mode)
     elif a.mode == "gpu-event":
-        gpu_event(P[0], a.duration)
+        gpu_event(P[0], a.duration, mode=a.surface_mode)
     elif a.mode == "gates":
-        gates(P[0], P[1], P[2])
+        gates(P[0], P[1], P[2], mode=a.surface_mode)
     elif a.mode == "oracle-300":
         oracle_300(P[0], a.threads, mode=a.surface_mode)
     elif a.mode == "gpu-300":
