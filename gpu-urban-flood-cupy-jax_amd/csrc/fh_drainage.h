@@ -42,10 +42,10 @@ struct DrainDev {
     real *sur_factor, *overflow, *fallback, *wsum;
     int store_returns;      // k_return runs: k_node_update stores overflow, weight sum, fallback
     // node -> inlets, ascending inlet ("slots"): the inlet's cell and surcharge
-    // limit, and per step the scaled surcharge and net exchange (capture - surcharge)
+    // limit, and per step the net exchange (capture - surcharge)
     const int *ni_ptr, *ni_cell;
     const real* ni_surq;
-    real *ni_sur, *ni_net;
+    real* ni_net;
     const int *nl_ptr, *nl_ent;        // node -> incident links, ascending link, (l << 1) | is_to
     // node -> incident links, 'from' entries then 'to' entries (each ascending), with the far node
     const int *nl2_ptr, *nl2_ent, *nl2_oth;
@@ -82,13 +82,9 @@ struct DrainDev {
     const real* rcf_sw;
     const int* rcf_fb;
     const real *ic_area, *rc_area;   // cell areas of the inlet cells and of the return cells, in list order
-    // links
-    const int *from, *to, *geom;
-    const real *len, *man, *lcap, *linv, *D, *minor, *rw, *rh;
+    // links: geometry code (2: rectangle), flow, this step's requested transfer
+    const int* geom;
     real *Q, *req;
-    // entries
-    const int *snode;
-    const real* sw;
     const int* snorm;        // [members] surcharge weights already normalized
     // return cells: entries (node, weight), fallback nodes, recharge rows
     const int *rc_cells, *rc_eptr, *rc_enode, *rc_fptr, *rc_fidx, *rc_rptr, *rc_ridx;
@@ -113,11 +109,6 @@ struct DrainDev {
 template <typename real>
 __device__ __forceinline__ real head_of(real inv, real As, real vol) {
     return inv + vmax(vol, real(0)) / vmax(As, real(1.0e-12));
-}
-
-template <typename real>
-__device__ __forceinline__ real node_head(const DrainDev<real>& d, int n, real vol) {
-    return head_of(d.inv[n], d.As[n], vol);
 }
 
 template <typename real>

@@ -1539,7 +1539,7 @@ public:
             }
             d_ni_cell.set(ni_cell, s); d_ni_surq.set(ni_surq, s);
             std::vector<real> zS((size_t)nslots, 0);
-            d_ni_sur.set(zS, s); d_ni_net.set(zS, s);
+            d_ni_net.set(zS, s);
             n_inlet_slots = nslots;
             d_ic_node.set(ic_node, s); d_ic_slot.set(ic_slot, s); d_ic_capq.set(ic_capq, s);
             d_nl2_ptr.set(nl2_ptr, s); d_nl2_ent.set(nl2_ent, s); d_nl2_oth.set(nl2_oth, s);
@@ -1549,12 +1549,10 @@ public:
         d_ni_ptr.set(ni_ptr, s); d_nl_ptr.set(nl_ptr, s); d_nl_ent.set(nl_ent, s);
         d_ne_ptr.set(ne_ptr, s);
         d_ic_cells.set(ic_cells, s); d_ic_ptr.set(ic_ptr, s);
-        d_from.set(from, s); d_to.set(to, s); d_geom.set(geom, s); d_llen.set(llen, s); d_man.set(man, s);
-        d_lcap.set(lcap, s); d_linv.set(linv, s); d_D.set(D, s); d_minor.set(minor, s); d_rw.set(rw, s);
-        d_rh.set(rh, s); d_Q.set(Q, s);
+        d_geom.set(geom, s); d_Q.set(Q, s);
         std::vector<real> zL(LT, 0);
         d_req.set(zL, s);
-        d_snode.set(snode, s); d_sw.set(sw, s); d_snorm.set(by_id(snorm, order), s);
+        d_snorm.set(by_id(snorm, order), s);
         d_rc_cells.set(rc_cells, s); d_rc_eptr.set(rc_eptr, s);
         d_rc_fptr.set(rc_fptr, s); d_rc_fidx.set(rc_fidx, s); d_rc_rptr.set(rc_rptr, s); d_rc_ridx.set(rc_ridx, s);
         d_rS.set(rS, s); d_rcc.set(rcc, s); d_rrc.set(rrc, s); d_rstop.set(rstop, s); d_rstore.set(rstore, s);
@@ -1580,7 +1578,7 @@ public:
         d.V = d_V.p; d.Vp = d_Vp.p; d.sur_factor = d_sfac.p;
         d.overflow = d_over.p;
         d.fallback = d_fb.p; d.wsum = d_wsum.p; d.store_returns = rxl.blocks > 0 ? 1 : 0;
-        d.ni_ptr = d_ni_ptr.p; d.ni_cell = d_ni_cell.p; d.ni_surq = d_ni_surq.p; d.ni_sur = d_ni_sur.p;
+        d.ni_ptr = d_ni_ptr.p; d.ni_cell = d_ni_cell.p; d.ni_surq = d_ni_surq.p;
         d.ni_net = d_ni_net.p; d.nl_ptr = d_nl_ptr.p; d.nl_ent = d_nl_ent.p;
         d.nl2_ptr = d_nl2_ptr.p; d.nl2_ent = d_nl2_ent.p; d.nl2_oth = d_nl2_oth.p;
         d.ne_ptr = d_ne_ptr.p; d.ne_sw = d_ne_sw.p;
@@ -1596,9 +1594,7 @@ public:
         d.nesw0 = d_nesw0.p;
         d.rcf_node = d_rcf_node.p; d.rcf_sw = d_rcf_sw.p; d.rcf_fb = d_rcf_fb.p;
         d.ic_area = d_ic_area.p; d.rc_area = d_rc_area.p;
-        d.from = d_from.p; d.to = d_to.p; d.geom = d_geom.p; d.len = d_llen.p; d.man = d_man.p; d.lcap = d_lcap.p;
-        d.linv = d_linv.p; d.D = d_D.p; d.minor = d_minor.p; d.rw = d_rw.p; d.rh = d_rh.p; d.Q = d_Q.p;
-        d.req = d_req.p; d.snode = d_snode.p; d.sw = d_sw.p; d.snorm = d_snorm.p;
+        d.geom = d_geom.p; d.Q = d_Q.p; d.req = d_req.p; d.snorm = d_snorm.p;
         d.rc_cells = d_rc_cells.p; d.rc_eptr = d_rc_eptr.p; d.rc_enode = d_rc_enode.p; d.rc_esw = d_rc_esw.p;
         d.rc_fptr = d_rc_fptr.p;
         d.rc_fidx = d_rc_fidx.p; d.rc_rptr = d_rc_rptr.p; d.rc_ridx = d_rc_ridx.p;
@@ -2124,8 +2120,10 @@ public:
         b += d_area.bytes() + d_n2.bytes() + d_lc.bytes() + d_rainm.bytes() + d_f0.bytes() + d_fc.bytes() + d_k.bytes();
         b += d_cls.bytes() + d_cls_tab.bytes() + d_cls_hk.bytes();
         b += d_ce0.bytes() + d_ce1.bytes() + d_ce2.bytes();
+        // drainage: the node, slot, entry and inlet tables by their representative
+        // arrays, the links by their two records, flow, request, code and work items
         b += d_inv.bytes() * 12 + d_ni_cell.bytes() * 4 + d_nl_ent.bytes() * 3 + d_ic_node.bytes() * 3 +
-             d_from.bytes() * 16;
+             d_lk_a.bytes() + d_lk_b.bytes() + d_Q.bytes() * 2 + d_geom.bytes() + d_it_l.bytes() * 7;
         return b;
     }
 
@@ -2173,11 +2171,10 @@ public:
     DevArray<V> d_outfall;
     DevArray<int> d_nflag, d_ni_ptr, d_ni_cell, d_nl_ptr, d_nl_ent, d_nl2_ptr, d_nl2_ent, d_nl2_oth, d_ne_ptr;
     DevArray<int> d_ic_cells, d_ic_ptr, d_ic_node, d_ic_slot;
-    DevArray<real> d_ni_surq, d_ni_sur, d_ni_net, d_ic_capq, d_ne_sw;
-    DevArray<int> d_from, d_to, d_geom;
-    DevArray<real> d_llen, d_man, d_lcap, d_linv, d_D, d_minor, d_rw, d_rh, d_Q, d_req;
-    DevArray<int> d_snode, d_snorm;
-    DevArray<real> d_sw;
+    DevArray<real> d_ni_surq, d_ni_net, d_ic_capq, d_ne_sw;
+    DevArray<int> d_geom;
+    DevArray<real> d_Q, d_req;
+    DevArray<int> d_snorm;
     DevArray<int> d_rc_cells, d_rc_eptr, d_rc_enode, d_rc_fptr, d_rc_fidx, d_rc_rptr, d_rc_ridx;
     DevArray<real> d_rc_esw;
     DevArray<int> d_nle, d_nl2e, d_nl2o, d_rcf_node, d_rcf_fb;

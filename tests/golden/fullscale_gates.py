@@ -13,11 +13,13 @@ host cores) and a GPU half (the HIP solver), joined by a comparison on the CPU:
    dt falls to about 0.05 s: the hour takes about 72,700 steps, so both sides run
    with a budget of 400,000 steps per output interval (the reference runner's
    default is 20,000).  The oracle half takes about an hour on 8 cores; it
-   checkpoints every 2,000 steps and resumes from the last checkpoint.
+   checkpoints every 2,000 steps and resumes from the last checkpoint.  With
+   --surface-mode local_inertial both halves run the local-inertial step (the C
+   restatement covers both modes).
 
-       python tests/golden/fullscale_gates.py oracle-event DIR [--threads N]
-       python tests/golden/fullscale_gates.py gpu-event OUT.npz            (GPU)
-       python tests/golden/fullscale_gates.py gates DIR/oracle_event.npz OUT.npz SUMMARY.json
+       python tests/golden/fullscale_gates.py oracle-event DIR [--threads N] [--surface-mode M]
+       python tests/golden/fullscale_gates.py gpu-event OUT.npz [--surface-mode M]   (GPU)
+       python tests/golden/fullscale_gates.py gates DIR/oracle_event.npz OUT.npz SUMMARY.json [--surface-mode M]
 
 2. 300 s of the dry-start event under the CFL policy, in float32 and in FP64 on
    both sides, for the drainage node volumes: FP64 against FP64 separates
@@ -33,7 +35,7 @@ host cores) and a GPU half (the HIP solver), joined by a comparison on the CPU:
    and compares one step at a time (the one-step error); trace-300 follows both
    FP64 trajectories step by step (depths, node volumes, outfall nodes).  With
    --surface-mode local_inertial the 300 s procedures run the local-inertial
-   step, against the NumPy oracle (the C restatement covers the full step).
+   step, against the NumPy oracle (oracle/local_inertial_np.py).
 """
 
 from __future__ import annotations
