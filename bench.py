@@ -119,9 +119,12 @@ def batch_leg(base, args, plain_rate):
     from floodhip import synth
     from floodhip.autotune import autotune
     members = synth.make_shared_mesh_ensemble(base, n_members=args.batch_members)
-    opts = dict(compensated=True, use_graph=True)
+    opts = dict(compensated=True, use_graph=not args.no_graph)
     tuned = None
-    if not args.no_autotune:
+    if args.batch_options:
+        bs, ept, order = (int(v) for v in args.batch_options.split(","))
+        opts.update(block_size=bs, edges_per_thread=ept, pack_order=order)
+    elif not args.no_autotune:
         # tuned on the whole batch: a shared-mesh batch's block order and cache
         # reuse depend on its member count, so a sample of members mispredicts
         best, table = autotune(members, steps=32, warmup=8, log=log, compensated=True)
@@ -130,7 +133,8 @@ def batch_leg(base, args, plain_rate):
     log(f"batched ensemble: {len(members)} members, options {opts}")
     rate, el, steps, prof = _ensemble_rate(members, args.batch_steps, 20, opts)
     # sequential plain baseline with the same state precision: one member alone, default launch options
-    seq_rate, _, _, _ = _ensemble_rate([members[0]], args.batch_steps, 20, dict(compensated=True, use_graph=True))
+    seq_rate, _, _, _ = _ensemble_rate([members[0]], args.batch_steps, 20,
+                                       dict(compensated=True, use_graph=not args.no_graph))
     log(f"batched: {rate:.4e} tri-steps/s over {steps} member-steps in {el:.3f}s; sequential plain "
         f"{seq_rate:.4e} ({rate / seq_rate:.2f}x)")
     # accuracy of the same batch against its FP64 run: 120 fixed 0.5 s steps
@@ -255,6 +259,9 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=200)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--batch-options", default=None,
+                    help="block size, edges per thread, pack order of the shared-mesh batch (e.g. 256,2,0) "
+                         "instead of autotuning them")
     ap.add_argument("--no-uneven", action="store_true", help="skip the 48-member uneven ensemble")
     ap.add_argument("--no-li", action="store_true", help="skip the local-inertial surface mode leg")
     ap.add_argument("--li-steps", type=int, default=1000)
@@ -335,14 +342,22 @@ def main():
     uneven = None if (args.no_uneven or args.no_batch) else batch_uneven_leg(args)
     copy = None
     if args.copy_probe_mib > 0:
+        # achievable bandwidth: a streaming copy, one 16-byte element per thread, with
+        # ordinary and with non-temporal loads and stores
         import ctypes
         from floodhip import _lib
         nf = args.copy_probe_mib * 2**20 // 4
-        ms = ctypes.c_double()
-        if _lib.load().fh_copy_probe(nf, 20, ctypes.byref(ms)) == 0:
-            copy = {"bytes_per_launch": 2 * nf * 4, "avg_us": 1e3 * ms.value,
-                    "GBps": 2 * nf * 4 / (ms.value * 1e-3) / 1e9}
-            log(f"copy probe: {copy['GBps']:.1f} GB/s ({copy['bytes_per_launch'] / 2**20:.0f} MiB per launch)")
+        copy = {"bytes_per_launch": 2 * nf * 4}
+        for key, nt in (("ordinary", 0), ("nontemporal", 1)):
+            ms = ctypes.c_double()
+            if _lib.load().fh_copy_probe(nf, 20, nt, ctypes.byref(ms)) == 0:
+                copy[key] = {"avg_us": 1e3 * ms.value, "GBps": 2 * nf * 4 / (ms.value * 1e-3) / 1e9}
+                log(f"copy probe ({key}): {copy[key]['GBps']:.1f} GB/s "
+                    f"({copy['bytes_per_launch'] / 2**20:.0f} MiB per launch)")
+        best = max((v["GBps"] for k, v in copy.items() if isinstance(v, dict)), default=None)
+        if best:
+            copy["GBps"] = best
+            roofline["frac_of_best_copy"] = achieved / best
     cpu = None
     if not args.no_cpu and args.cpu_steps > 0:
         log(f"CPU baseline: oracle, {args.cpu_steps} steps of the same workload")

@@ -70,10 +70,11 @@ __device__ __forceinline__ EdgeFlux<real> edge_flux(real h_l, real hu_l, real hv
     const real hs_l = vmax(h_l + z_l - z_star, zero);
     const real hs_r = vmax(h_r + z_r - z_star, zero);
     const bool wl0 = h_l > eps, wr0 = h_r > eps;
-    const real u_l0 = wl0 ? hu_l / vmax(h_l, eps) : zero;
-    const real v_l0 = wl0 ? hv_l / vmax(h_l, eps) : zero;
-    const real u_r0 = wr0 ? hu_r / vmax(h_r, eps) : zero;
-    const real v_r0 = wr0 ? hv_r / vmax(h_r, eps) : zero;
+    const Divisor<real> dl0(vmax(h_l, eps)), dr0(vmax(h_r, eps));
+    const real u_l0 = wl0 ? dl0.div(hu_l) : zero;
+    const real v_l0 = wl0 ? dl0.div(hv_l) : zero;
+    const real u_r0 = wr0 ? dr0.div(hu_r) : zero;
+    const real v_r0 = wr0 ? dr0.div(hv_r) : zero;
     const real qhu_l = u_l0 * hs_l, qhv_l = v_l0 * hs_l;
     const real qhu_r = u_r0 * hs_r, qhv_r = v_r0 * hs_r;
     // _hll on the reconstructed states
@@ -81,10 +82,11 @@ __device__ __forceinline__ EdgeFlux<real> edge_flux(real h_l, real hu_l, real hv
     const bool wl = a_l > eps, wr = a_r > eps;
     real fh = zero, fhu = zero, fhv = zero;
     if (wl || wr) {
-        const real ul = wl ? qhu_l / vmax(a_l, eps) : zero;
-        const real vl = wl ? qhv_l / vmax(a_l, eps) : zero;
-        const real ur = wr ? qhu_r / vmax(a_r, eps) : zero;
-        const real vr = wr ? qhv_r / vmax(a_r, eps) : zero;
+        const Divisor<real> dl(vmax(a_l, eps)), dr(vmax(a_r, eps));
+        const real ul = wl ? dl.div(qhu_l) : zero;
+        const real vl = wl ? dl.div(qhv_l) : zero;
+        const real ur = wr ? dr.div(qhu_r) : zero;
+        const real vr = wr ? dr.div(qhv_r) : zero;
         const real un_l = ul * nx + vl * ny;
         const real un_r = ur * nx + vr * ny;
         const real c_l = sqrt(p.g * a_l), c_r = sqrt(p.g * a_r);
@@ -99,11 +101,12 @@ __device__ __forceinline__ EdgeFlux<real> edge_flux(real h_l, real hu_l, real hv
             fh = fr0; fhu = fr1; fhv = fr2;
         } else {
             const real ds = s_r - s_l;
-            const real denom = fabs(ds) > p.tiny ? ds : real(1);
+            // s_l < 0 < s_r here, so the divisor is positive
+            const Divisor<real> denom(fabs(ds) > p.tiny ? ds : real(1));
             const real slsr = s_l * s_r;
-            fh = (s_r * fl0 - s_l * fr0 + slsr * (a_r - a_l)) / denom;
-            fhu = (s_r * fl1 - s_l * fr1 + slsr * (qhu_r - qhu_l)) / denom;
-            fhv = (s_r * fl2 - s_l * fr2 + slsr * (qhv_r - qhv_l)) / denom;
+            fh = denom.div(s_r * fl0 - s_l * fr0 + slsr * (a_r - a_l));
+            fhu = denom.div(s_r * fl1 - s_l * fr1 + slsr * (qhu_r - qhu_l));
+            fhv = denom.div(s_r * fl2 - s_l * fr2 + slsr * (qhv_r - qhv_l));
         }
     }
     const real pc_l = p.half_g * (h_l * h_l - hs_l * hs_l);
@@ -137,10 +140,11 @@ struct ForcingTables {
 template <typename real>
 __device__ __forceinline__ real cell_cfl(const KP<real>& p, const typename V4<real>::type& s, real lc) {
     const real hs = vmax(s.x, p.eps);
-    const real u = s.y / hs, v = s.z / hs;
+    const Divisor<real> dh(hs);
+    const real u = dh.div(s.y), v = dh.div(s.z);
     real speed = sqrt(u * u + v * v);
     speed = speed + sqrt(p.g * hs);
-    const real local = p.cfl * lc / vmax(speed, p.tiny);
+    const real local = Divisor<real>(vmax(speed, p.tiny)).div(p.cfl * lc);
     return isfinite(local) ? local : real(INFINITY);
 }
 
@@ -151,7 +155,7 @@ template <typename real>
 __device__ __forceinline__ real cell_cfl_li(const KP<real>& p, const typename V4<real>::type& s, real lc) {
     const real hs = vmax(s.x, p.eps);
     const real c = sqrt(p.g * hs);
-    const real local = p.cfl * lc / vmax(c, p.tiny);
+    const real local = Divisor<real>(vmax(c, p.tiny)).div(p.cfl * lc);
     return isfinite(local) ? local : real(INFINITY);
 }
 
@@ -567,7 +571,7 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
         const real bed_star = interior ? vmax(sl.w, sr.w) : sl.w;
         const real h_flow = vmax(vmax(eta_l, eta_r) - bed_star, zero);
         const real distance = vmax(half * (lcl + (interior ? lcr : lcl)), real(1.0e-6));
-        const real slope = (eta_r - eta_l) / distance;
+        const real slope = Divisor<real>(distance).div(eta_r - eta_l);
         const real nm_l = sl.y * nx + sl.z * ny;
         real nm_face = nm_l;
         real n_face = nl;
@@ -581,8 +585,9 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
         const real radius = vmax(h_flow, real(1.0e-6));
         const real numerator = q_old - p.g * area * dt * slope;
         const real gdt = (real)(p.g_d * pl.dt);
-        const real denominator = real(1) + (gdt * (n_face * n_face) * fabs(q_old) / (pow(radius, p.pow43) * area));
-        q = numerator / denominator;
+        const real denominator =
+            real(1) + Divisor<real>(pow(radius, p.pow43) * area).div(gdt * (n_face * n_face) * fabs(q_old));
+        q = Divisor<real>(denominator).div(numerator);
         if (!interior) q = vmax(q, zero);
     }
     V o;
@@ -652,7 +657,7 @@ __global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<rea
             sum = sum + fabs(r);
         }
     }
-    const real factor = vmin(real(1), (vmax(h, zero) * A) / vmax(sum, p.tiny));
+    const real factor = vmin(real(1), Divisor<real>(vmax(sum, p.tiny)).div(vmax(h, zero) * A));
 #pragma unroll
     for (int k = 0; k < 3; ++k)
         if (donor[k]) T[(ent[k] >> 2) + eb] = (req[k] >= zero ? real(1) : real(-1)) * (fabs(req[k]) * factor);
@@ -811,7 +816,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
                     req[k] = (en >= 0 && (en & 2)) ? vmax(rr[k].x, zero) * dt : zero;
                     if (en >= 0 && (en & 2)) bsum = bsum + req[k];
                 }
-                const real bfac = vmin(real(1), vol_after / vmax(bsum, p.tiny));
+                const real bfac = vmin(real(1), Divisor<real>(vmax(bsum, p.tiny)).div(vol_after));
                 real out = zero;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
@@ -823,10 +828,10 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
                 }
                 const real vdelta = delta - out;
                 if (COMP) {
-                    h = two_sum<real>(s.x, vdelta / A + cc.x, ch);
+                    h = two_sum<real>(s.x, Divisor<real>(A).div(vdelta) + cc.x, ch);
                     if (!(h > zero)) { h = vmax(h, zero); ch = zero; }
                 } else {
-                    h = vmax(s.x + vdelta / A, zero);
+                    h = vmax(s.x + Divisor<real>(A).div(vdelta), zero);
                 }
                 // momentum from the edge discharges around the cell
                 // (triangular_backend.py:1270-1285): from entries, then to entries
@@ -838,10 +843,11 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
                     if (en & 1) { Ru = Ru + rr[k].y; Rv = Rv + rr[k].z; }
                     else { Lu = Lu + rr[k].y; Lv = Lv + rr[k].z; }
                 }
-                const real perimeter = real(2) * A / vmax(lcc, p.tiny);
+                const real perimeter = Divisor<real>(vmax(lcc, p.tiny)).div(real(2) * A);
                 const bool w = h > eps;
-                hu = w ? (Lu + Ru) / vmax(perimeter, p.tiny) : zero;
-                hv = w ? (Lv + Rv) / vmax(perimeter, p.tiny) : zero;
+                const Divisor<real> dp(vmax(perimeter, p.tiny));
+                hu = w ? dp.div(Lu + Ru) : zero;
+                hv = w ? dp.div(Lv + Rv) : zero;
             } else {
             real Lh = zero, Lu = zero, Lv = zero, Rh = zero, Ru = zero, Rv = zero;
             bool anyR = false;
@@ -864,25 +870,30 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
             const real dh = anyR ? Lh + Rh : Lh;
             const real dhu = anyR ? Lu + Ru : Lu;
             const real dhv = anyR ? Lv + Rv : Lv;
+            const Divisor<real> dA(A);
             if (COMP) {
-                h = two_sum<real>(s.x, dt * dh / A + cc.x, ch);
-                hu = two_sum<real>(s.y, dt * dhu / A + cc.y, chu);
-                hv = two_sum<real>(s.z, dt * dhv / A + cc.z, chv);
+                h = two_sum<real>(s.x, dA.div(dt * dh) + cc.x, ch);
+                hu = two_sum<real>(s.y, dA.div(dt * dhu) + cc.y, chu);
+                hv = two_sum<real>(s.z, dA.div(dt * dhv) + cc.z, chv);
                 if (!(h > zero)) { h = vmax(h, zero); ch = zero; }
             } else {
-                h = vmax(s.x + dt * dh / A, zero);
-                hu = s.y + dt * dhu / A;
-                hv = s.z + dt * dhv / A;
+                h = vmax(s.x + dA.div(dt * dh), zero);
+                hu = s.y + dA.div(dt * dhu);
+                hv = s.z + dA.div(dt * dhv);
             }
             // Manning friction (semi-implicit)
             {
                 const bool w = h > eps;
                 const real hs = vmax(h, eps);
-                const real u = hu / hs, v = hv / hs;
+                const Divisor<real> dhs(hs);
+                const real u = dhs.div(hu), v = dhs.div(hv);
                 const real speed = sqrt(u * u + v * v);
-                const real damping = real(1) + dt * p.g * n2 * speed / pow(hs, p.pow43);
-                if (w) { hu = hu / damping; hv = hv / damping; if (COMP) { chu = chu / damping; chv = chv / damping; } }
-                else { hu = zero; hv = zero; chu = zero; chv = zero; }
+                const real damping = real(1) + Divisor<real>(pow(hs, p.pow43)).div(dt * p.g * n2 * speed);
+                if (w) {
+                    const Divisor<real> dd(damping);
+                    hu = dd.div(hu); hv = dd.div(hv);
+                    if (COMP) { chu = dd.div(chu); chv = dd.div(chv); }
+                } else { hu = zero; hv = zero; chu = zero; chv = zero; }
             }
             }
             // rain + Horton infiltration (capacity from the wet time of the previous step)
@@ -893,13 +904,13 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
             const real infil = vmin(h_before, vmax(capacity, zero) * dt);
             real h2, e2 = zero;
             if (COMP) h2 = two_sum<real>(h_before, -infil, e2); else h2 = h_before - infil;
-            const real scale = h_before > eps ? h2 / vmax(h_before, eps) : zero;
+            const real scale = h_before > eps ? Divisor<real>(vmax(h_before, eps)).div(h2) : zero;
             if (h2 > eps) { hu = hu * scale; hv = hv * scale; if (COMP) { chu = chu * scale; chv = chv * scale; } }
             else { hu = zero; hv = zero; chu = zero; chv = zero; }
             // velocity clamp
             const bool w2 = h2 > eps;
-            const real speed2 = w2 ? sqrt(hu * hu + hv * hv) / vmax(h2, eps) : zero;
-            const real sc = vmin(real(1), p.max_speed / vmax(speed2, p.tiny));
+            const real speed2 = w2 ? Divisor<real>(vmax(h2, eps)).div(sqrt(hu * hu + hv * hv)) : zero;
+            const real sc = vmin(real(1), Divisor<real>(vmax(speed2, p.tiny)).div(p.max_speed));
             if (w2) { hu = hu * sc; hv = hv * sc; if (COMP) { chu = chu * sc; chv = chv * sc; } }
             else { hu = zero; hv = zero; chu = zero; chv = zero; }
             s.x = h2; s.y = hu; s.z = hv;

@@ -80,7 +80,8 @@ SIGNATURES = {
                                      C.POINTER(C.c_int64)]),
     "fh_device_bytes": (C.c_int64, [_ctx]),
     "fh_dry_weather_step": (C.c_int, [_ctx, C.c_double, _f, _f, C.c_int32, _d]),
-    "fh_copy_probe": (C.c_int, [C.c_int64, C.c_int32, _d]),
+    "fh_copy_probe": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, _d]),
+    "fh_division_selftest": (C.c_int, [C.POINTER(C.c_int64)]),
 }
 
 _LIB = None

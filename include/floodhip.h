@@ -242,11 +242,22 @@ int fh_profile_kernels(fh_ctx* ctx, int32_t steps, int32_t cap, int32_t* count,
 /* Size of the resident device footprint in bytes. */
 int64_t fh_device_bytes(const fh_ctx* ctx);
 
+/* Verification helper (no counterpart in the reference): checks on device 0
+ * the division the float32 kernels use (a shared, Newton-refined reciprocal and
+ * one remainder correction) against IEEE division.  counts[6]: reciprocal
+ * mismatches and reciprocals tested (every significand at nine exponents from
+ * 2^-100 to 2^99), quotient mismatches and quotients tested over random pairs
+ * in the exact domain (a = +-0, or 2^-100 <= |a|, |a / b| <= 2^100, with
+ * 2^-100 <= b <= 2^100), and the same two for dividends below 2^-100 (outside
+ * the domain: at most one ulp off). */
+int fh_division_selftest(int64_t* counts);
+
 /* Measurement helper (no counterpart in the reference): times `reps` launches
- * of a plain streaming copy of n_floats floats (n_floats % 4 == 0) on device 0
- * and returns the average milliseconds per launch.  Used as the reference
- * point for HBM byte counters and achievable bandwidth. */
-int fh_copy_probe(int64_t n_floats, int32_t reps, double* avg_ms);
+ * of a streaming copy of n_floats floats (n_floats % 4 == 0) on device 0, one
+ * 16-byte element per thread, with ordinary (nontemporal == 0) or non-temporal
+ * loads and stores, and returns the average milliseconds per launch.  Used as
+ * the reference point for HBM byte counters and achievable bandwidth. */
+int fh_copy_probe(int64_t n_floats, int32_t reps, int32_t nontemporal, double* avg_ms);
 
 #ifdef __cplusplus
 }
