@@ -133,6 +133,7 @@ __device__ __forceinline__ unsigned long long* cfl_slot(unsigned long long* base
 template <typename real>
 struct KP {                 // kernel constants, pre-rounded to the state precision
     real g, eps, half_g, two_g, cfl, max_speed, tiny, capture_q, pow43, reuse, slot_coef;
+    double pow43_dln2;       // ((double)pow43 - 4/3) ln 2, for fric_pow in float
     double max_dt, dt_floor, fixed_dt;
     double g_d;              // gravity in double (products of two host scalars, g * dt)
     int interval;
@@ -142,10 +143,18 @@ struct KP {                 // kernel constants, pre-rounded to the state precis
 
 // NaN-propagating max/min, like numpy.maximum / jax.numpy.maximum (fmax/fmin
 // would drop a NaN operand and hide a blown-up state from the finite checks).
+// In float they are gfx950's v_maximum3_f32 / v_minimum3_f32 (IEEE 754-2019
+// maximum / minimum), one instruction against a compare pair and a select; the
+// one difference from the select form is the sign of a zero result when the
+// operands are +0 and -0 (those order -0 below +0).
 template <typename T>
 __device__ __forceinline__ T vmax(T a, T b) { return (a != a || a > b) ? a : b; }
 template <typename T>
 __device__ __forceinline__ T vmin(T a, T b) { return (a != a || a < b) ? a : b; }
+template <>
+__device__ __forceinline__ float vmax(float a, float b) { return __builtin_elementwise_maximum(a, b); }
+template <>
+__device__ __forceinline__ float vmin(float a, float b) { return __builtin_elementwise_minimum(a, b); }
 
 // Division by a positive divisor b, rounded to nearest-even exactly as IEEE
 // a / b, with the reciprocal computed once per divisor and shared by every
@@ -180,6 +189,26 @@ struct Divisor<float> {
         return __builtin_fmaf(-t, y, q);
     }
 };
+
+// x^p for the friction exponent p = (float)(4/3) = 4/3 + d, d = 3.97e-8, as a
+// float32 program raises to "4/3", for x >= 2^-100 (the callers floor their
+// depth or radius).  x^p = (x r)^2 (1 + d ln x) with r = x^(-1/3): r0 from the
+// hardware log2 and exp2 (relative error ~1e-6), one Newton step
+// r = r0 (4 - x r0^3) / 3 in FP64 (error ~2 e0^2), then the product in FP64
+// (x^d - 1 - d ln x is below 1e-13) and one rounding to float.  That is the
+// correctly rounded x^p unless x^p lies within ~1e-12 relative of a rounding
+// boundary: over every float in [2^-40, 2^20) (fh_pow_selftest) it is one ulp
+// off for 4,284 of 503 M values, the device library's powf for 127 M.  About 17
+// instructions against 137.  FP64 runs keep pow.
+__device__ __forceinline__ float fric_pow(float x, double dln2) {
+    const float l2 = __builtin_amdgcn_logf(x);
+    const double r0 = (double)__builtin_amdgcn_exp2f(l2 * (-1.0f / 3.0f));
+    const double xd = (double)x;
+    const double r = r0 * (4.0 - xd * (r0 * r0 * r0)) * (1.0 / 3.0);
+    const double t = xd * r;
+    return (float)((t * t) * (1.0 + (double)l2 * dln2));
+}
+__device__ __forceinline__ double fric_pow(double x, double) { return pow(x, 4.0 / 3.0); }
 
 __device__ __forceinline__ int xcd_block(int i, int G, int remap) {
     // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a

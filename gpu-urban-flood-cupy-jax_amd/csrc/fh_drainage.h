@@ -393,7 +393,7 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const LinkData<r
     const real slope = (ha - hb) / length;
     const real nm = vmax(k.man, zero);
     const real absq = fabs(k.q0);
-    const real friction = g * (nm * nm) * absq / (pow(radius, p.pow43) * area);
+    const real friction = g * (nm * nm) * absq / (fric_pow(radius, p.pow43_dln2) * area);
     const real minor = vmax(k.minor, zero) * absq / (real(2) * length * area);
     real cand = (k.q0 + g * area * dt * slope) / (real(1) + dt * (friction + minor));
     const real cap = vmax(k.lcap, zero);

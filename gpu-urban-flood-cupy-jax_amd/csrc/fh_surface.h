@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
         const real numerator = q_old - p.g * area * dt * slope;
         const real gdt = (real)(p.g_d * pl.dt);
         const real denominator =
-            real(1) + Divisor<real>(pow(radius, p.pow43) * area).div(gdt * (n_face * n_face) * fabs(q_old));
+            real(1) + Divisor<real>(fric_pow(radius, p.pow43_dln2) * area).div(gdt * (n_face * n_face) * fabs(q_old));
         q = Divisor<real>(denominator).div(numerator);
         if (!interior) q = vmax(q, zero);
     }
@@ -888,7 +888,7 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
                 const Divisor<real> dhs(hs);
                 const real u = dhs.div(hu), v = dhs.div(hv);
                 const real speed = sqrt(u * u + v * v);
-                const real damping = real(1) + Divisor<real>(pow(hs, p.pow43)).div(dt * p.g * n2 * speed);
+                const real damping = real(1) + Divisor<real>(fric_pow(hs, p.pow43_dln2)).div(dt * p.g * n2 * speed);
                 if (w) {
                     const Divisor<real> dd(damping);
                     hu = dd.div(hu); hv = dd.div(hv);

@@ -395,6 +395,7 @@ public:
         kp.tiny = (real)1.0e-12;
         kp.capture_q = (real)P.inlet_orifice_q;
         kp.pow43 = (real)(4.0 / 3.0);
+        kp.pow43_dln2 = ((double)(float)(4.0 / 3.0) - 4.0 / 3.0) * M_LN2;
         kp.reuse = (real)P.reuse_factor;
         kp.slot_coef = (real)(P.slot_wave_speed * P.slot_wave_speed);
         kp.max_dt = P.max_dt;
@@ -2292,6 +2293,29 @@ __global__ void __launch_bounds__(256) k_div_pairs(unsigned long long* __restric
     div_count(c + 5, n_tiny);
 }
 
+// Device check of fh::fric_pow<float> (fh_pow_selftest): every float x in
+// [2^kPowExp0, 2^kPowExp1) against the correctly rounded x^p, p = (float)(4/3),
+// taken as the FP64 pow rounded to float; the device library's powf beside it.
+// c[0] fric_pow mismatches, c[1] values, c[2] powf mismatches, c[3] and c[4]
+// the largest ulp distance of each.
+constexpr int kPowExp0 = -40, kPowExp1 = 20;
+
+__global__ void __launch_bounds__(256) k_pow_check(unsigned long long* __restrict__ c, double dln2) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;   // < (kPowExp1 - kPowExp0) << 23 by the grid
+    const float x = __uint_as_float((uint32_t)(kPowExp0 + 127) * (1u << 23) + i);
+    const float p = (float)(4.0 / 3.0);
+    const float ref = (float)pow((double)x, (double)p);
+    const float mine = fh::fric_pow(x, dln2);
+    const float lib = powf(x, p);
+    const int dm = abs((int)__float_as_uint(mine) - (int)__float_as_uint(ref));
+    const int dl = abs((int)__float_as_uint(lib) - (int)__float_as_uint(ref));
+    div_count(c, dm ? 1u : 0u);
+    __syncthreads();
+    div_count(c + 2, dl ? 1u : 0u);
+    if (dm) atomicMax(&c[3], (unsigned long long)dm);
+    if (dl) atomicMax(&c[4], (unsigned long long)dl);
+}
+
 }  // namespace
 
 struct fh_ctx {
@@ -2626,6 +2650,29 @@ int fh_stable_timestep_noalloc(const float* h, const float* hu, const float* hv,
         return FH_ERR_RUNTIME;
     }
     return FH_OK;
+}
+
+int fh_pow_selftest(int64_t* counts) {
+    if (!counts) return FH_ERR_VALUE;
+    unsigned long long* c = nullptr;
+    int status = FH_OK;
+    if (hipMalloc((void**)&c, 8 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(c, 0, 8 * sizeof(unsigned long long)) != hipSuccess) {
+        status = FH_ERR_RUNTIME;
+    } else {
+        const unsigned long long n = (unsigned long long)(kPowExp1 - kPowExp0) << 23;
+        k_pow_check<<<(unsigned)(n / 256), 256>>>(c, ((double)(float)(4.0 / 3.0) - 4.0 / 3.0) * M_LN2);
+        unsigned long long h[8] = {};
+        if (hipGetLastError() != hipSuccess || hipMemcpy(h, c, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) {
+            status = FH_ERR_RUNTIME;
+        } else {
+            h[1] = n;
+            for (int k = 0; k < 5; ++k) counts[k] = (int64_t)h[k];
+        }
+    }
+    if (c) (void)hipFree(c);
+    if (status != FH_OK) g_error = "pow self-test failed to run";
+    return status;
 }
 
 int fh_division_selftest(int64_t* counts) {

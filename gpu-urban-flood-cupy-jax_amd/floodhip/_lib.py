@@ -82,6 +82,7 @@ SIGNATURES = {
     "fh_dry_weather_step": (C.c_int, [_ctx, C.c_double, _f, _f, C.c_int32, _d]),
     "fh_copy_probe": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, _d]),
     "fh_division_selftest": (C.c_int, [C.POINTER(C.c_int64)]),
+    "fh_pow_selftest": (C.c_int, [C.POINTER(C.c_int64)]),
 }
 
 _LIB = None

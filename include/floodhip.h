@@ -252,6 +252,14 @@ int64_t fh_device_bytes(const fh_ctx* ctx);
  * the domain: at most one ulp off). */
 int fh_division_selftest(int64_t* counts);
 
+/* Verification helper (no counterpart in the reference): checks on device 0
+ * the power x^p, p = (float)(4/3), that the float32 friction terms use (FP64
+ * Newton step on a hardware estimate) against the correctly rounded value over
+ * every float x in [2^-40, 2^20).  counts[5]: its mismatches, the values
+ * tested, the device library powf's mismatches on the same values, and the
+ * largest ulp distance of each. */
+int fh_pow_selftest(int64_t* counts);
+
 /* Measurement helper (no counterpart in the reference): times `reps` launches
  * of a streaming copy of n_floats floats (n_floats % 4 == 0) on device 0, one
  * 16-byte element per thread, with ordinary (nontemporal == 0) or non-temporal
