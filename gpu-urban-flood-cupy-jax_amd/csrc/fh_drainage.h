@@ -108,7 +108,7 @@ struct DrainDev {
 
 template <typename real>
 __device__ __forceinline__ real head_of(real inv, real As, real vol) {
-    return inv + vmax(vol, real(0)) / vmax(As, real(1.0e-12));
+    return inv + Divisor<real>(vmax(As, real(1.0e-12))).div(vmax(vol, real(0)));
 }
 
 template <typename real>
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(256) k_external(Ctl* __restrict__ ctls, KP<rea
                 q = ql + frac * (qu - ql);
             }
             const real added = vmax(q, real(0)) * dt;
-            acc = acc + added / A;
+            acc = acc + Divisor<real>(A).div(added);
             added_sum += (double)added;
         }
         state[c].x = state[c].x + acc;
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(kInletThreads) k_inlet(Ctl* __restrict__ ctls,
             total = total + rs;
             if (sib0 + u == slot) own = rs;
         }
-        lsur[t] = own * vmin(real(1), Vn / vmax(total, real(1.0e-12)));
+        lsur[t] = own * vmin(real(1), Divisor<real>(vmax(total, real(1.0e-12))).div(Vn));
         const real q = orifice_q(p, (sc.w + sc.x) - nh);
         lrc[t] = vmin(vmax(q, real(0)) * dt, vmax(capq, real(0)) * dt);
     }
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(kInletThreads) k_inlet(Ctl* __restrict__ ctls,
             total = total + rs;
             if (k == si) own = rs;
         }
-        lsur[i - e0] = own * vmin(real(1), Vi / vmax(total, real(1.0e-12)));
+        lsur[i - e0] = own * vmin(real(1), Divisor<real>(vmax(total, real(1.0e-12))).div(Vi));
         const V si_s = state[d.ie_cell[i]];
         const real q = orifice_q(p, (si_s.w + si_s.x) - nh);
         lrc[i - e0] = vmin(vmax(q, real(0)) * dt, vmax(d.ic_capq[i], real(0)) * dt);
@@ -282,12 +282,12 @@ __global__ void __launch_bounds__(kInletThreads) k_inlet(Ctl* __restrict__ ctls,
     if (row_thread && j < mend) {
         real total = real(0);
         for (int k = k0; k < k1; ++k) total = total + lrc[k - e0];
-        const real factor = vmin(real(1), (h0 * A) / vmax(total, real(1.0e-12)));
+        const real factor = vmin(real(1), Divisor<real>(vmax(total, real(1.0e-12))).div(h0 * A));
         real acc = real(0);
         for (int k = k0; k < k1; ++k) {
             const real cp = lrc[k - e0] * factor;
             const real sr = lsur[k - e0];
-            acc = acc + (sr - cp) / A;
+            acc = acc + Divisor<real>(A).div(sr - cp);
             cap_sum += (double)cp;
             sur_sum += (double)sr;
         }
@@ -340,7 +340,7 @@ template <typename real>
 __device__ __forceinline__ void circular_theta(real depth, real diameter, real& radius, real& theta) {
     radius = real(0.5) * diameter;
     const real od = vmin(vmax(depth, real(0)), diameter);
-    real x = (radius - od) / radius;
+    real x = Divisor<real>(radius).div(radius - od);
     x = vmin(vmax(x, real(-1)), real(1));
     theta = real(2) * acos(x);
 }
@@ -372,7 +372,7 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const LinkData<r
     const real depth = vmax(real(0.5) * (ha + hb) - k.linv, zero);
     const real Draw = k.D;
     const real Dm = vmax(Draw, real(1.0e-12));
-    const real slot_width = g * (real(0.25 * M_PI) * (Dm * Dm)) / p.slot_coef;
+    const real slot_width = Divisor<real>(p.slot_coef).div(g * (real(0.25 * M_PI) * (Dm * Dm)));
     real radius_c, theta;
     circular_theta(depth, Dm, radius_c, theta);
     const real open_c = real(0.5) * (radius_c * radius_c) * (theta - sin(theta));
@@ -380,22 +380,22 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const LinkData<r
     const real circ_area = depth >= Dm ? full : open_c;
     real flow_area = circ_area + slot_width * vmax(depth - Draw, zero);
     const real perim = depth >= Dm ? real(M_PI) * Dm : radius_c * theta;
-    real radius = circ_area / vmax(perim, real(1.0e-12));
+    real radius = Divisor<real>(vmax(perim, real(1.0e-12))).div(circ_area);
     if (k.geom == 2) {
         const real od = vmin(vmax(depth, zero), vmax(k.rh, real(1.0e-6)));
         const real oa = vmax(k.rw, real(1.0e-6)) * od;
         flow_area = oa;
-        radius = oa / vmax(k.rw + real(2) * od, real(1.0e-12));
+        radius = Divisor<real>(vmax(k.rw + real(2) * od, real(1.0e-12))).div(oa);
     }
     const real length = vmax(k.len, real(1.0e-6));
     const real area = vmax(flow_area, real(1.0e-12));
     radius = vmax(radius, real(1.0e-12));
-    const real slope = (ha - hb) / length;
+    const real slope = Divisor<real>(length).div(ha - hb);
     const real nm = vmax(k.man, zero);
     const real absq = fabs(k.q0);
-    const real friction = g * (nm * nm) * absq / (fric_pow(radius, p.pow43_dln2) * area);
-    const real minor = vmax(k.minor, zero) * absq / (real(2) * length * area);
-    real cand = (k.q0 + g * area * dt * slope) / (real(1) + dt * (friction + minor));
+    const real friction = Divisor<real>(fric_pow(radius, p.pow43_dln2) * area).div(g * (nm * nm) * absq);
+    const real minor = Divisor<real>(real(2) * length * area).div(vmax(k.minor, zero) * absq);
+    real cand = Divisor<real>(real(1) + dt * (friction + minor)).div(k.q0 + g * area * dt * slope);
     const real cap = vmax(k.lcap, zero);
     cand = vmin(vmax(cand, -cap), cap);
     return cand * dt;
@@ -503,7 +503,7 @@ __global__ void __launch_bounds__(kLinkThreads) k_link_node(const Ctl* __restric
             if (positive != is_to) total = total + fabs(r);   // donor = from if positive else to
         }
     }
-    d.sur_factor[n] = vmin(real(1), vn / vmax(total, real(1.0e-12)));
+    d.sur_factor[n] = vmin(real(1), Divisor<real>(vmax(total, real(1.0e-12))).div(vn));
 }
 
 // Node continuity (one thread per node): donor-limited link transfers
@@ -578,7 +578,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
                     const real T = (positive ? real(1) : real(-1)) * fabs(r[u]) * f;
                     if (!is_to) {
                         delta = delta + (-T);
-                        d.Q[en[u] >> 1] = T / vmax(dt, real(1.0e-12));
+                        d.Q[en[u] >> 1] = Divisor<real>(vmax(dt, real(1.0e-12))).div(T);
                         if (dry) moved += (double)fabs(T);
                     } else {
                         delta = delta + T;
@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
             for (int k = e0; k < e1; ++k) wsum = wsum + vmax(k == e0 ? sw0 : d.ne_sw[k], zero);
             for (int k = e0; k < e1; ++k) {
                 const real raw = vmax(k == e0 ? sw0 : d.ne_sw[k], zero);
-                const real norm = d.snorm[mem] ? raw : raw / vmax(wsum, real(1.0e-12));
+                const real norm = d.snorm[mem] ? raw : Divisor<real>(vmax(wsum, real(1.0e-12))).div(raw);
                 returned = returned + over * norm;
             }
             returned = vmin(returned, over);
@@ -635,7 +635,7 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
         // h + (0 + fb / A) leaves h's bits unchanged when fb is +0, unless h is -0
         if (rcell >= 0 && (fb != zero || (rh == zero && signbit(rh)))) {
             real acc = zero;
-            acc = acc + fb / rA;
+            acc = acc + Divisor<real>(rA).div(fb);
             state[rcell].x = rh + acc;
         }
         out_sum = (double)outf;
@@ -689,14 +689,14 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
                     const int n = d.rc_enode[k];
                     raw = vmax(d.rc_esw[k], zero); ov = d.overflow[n]; ws = d.wsum[n];
                 }
-                const real norm = d.snorm[mem] ? raw : raw / vmax(ws, real(1.0e-12));
-                acc = acc + (ov * norm) / A;
+                const real norm = d.snorm[mem] ? raw : Divisor<real>(vmax(ws, real(1.0e-12))).div(raw);
+                acc = acc + Divisor<real>(A).div(ov * norm);
             }
             h = h + acc;
         }
         {
             real acc = zero;
-            for (int k = kf0; k < kf1; ++k) acc = acc + (k == kf0 ? fb0 : d.fallback[d.rc_fidx[k]]) / A;
+            for (int k = kf0; k < kf1; ++k) acc = acc + Divisor<real>(A).div(k == kf0 ? fb0 : d.fallback[d.rc_fidx[k]]);
             h = h + acc;
         }
         if (d.R > 0 && d.rc_rptr[j + 1] > d.rc_rptr[j]) {
@@ -714,7 +714,7 @@ __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real>
                 cap_sum += (double)capt;
                 loss_sum += (double)loss;
             }
-            h = vmax(h - removed / A, zero);
+            h = vmax(h - Divisor<real>(A).div(removed), zero);
         }
         state[c].x = h;
     }
