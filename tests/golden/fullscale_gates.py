@@ -35,7 +35,7 @@ host cores) and a GPU half (the HIP solver), joined by a comparison on the CPU:
    and compares one step at a time (the one-step error); trace-300 follows both
    FP64 trajectories step by step (depths, node volumes, outfall nodes).  With
    --surface-mode local_inertial the 300 s procedures run the local-inertial
-   step, against the NumPy oracle (oracle/local_inertial_np.py).
+   step (the C restatement covers both modes).
 """
 
 from __future__ import annotations
@@ -191,23 +191,18 @@ def _tag(dtype_tag: str, mode: str) -> str:
 
 
 class _Oracle:
-    """The oracle's CFL-policy advance: the C restatement for the full step, the
-    NumPy oracle for the local-inertial step (which the C restatement does not
-    cover)."""
+    """The oracle's CFL-policy advance: the C restatement on the host cores, in
+    either surface mode (tests/test_oracle_c.py pins it against the NumPy oracle
+    step by step in both)."""
 
     def __init__(self, io, dtype, mode):
+        from oracle import v1_parity_c as oc
         self.io, self.dtype, self.mode = io, dtype, mode
-        if mode == "full_swe":
-            from oracle import v1_parity_c as oc
-            s, c, p, r, _, f = io
-            self.co = oc.COracle(s, c, p, r, f, dtype=dtype)
+        s, c, p, r, _, f = io
+        self.co = oc.COracle(s, c, p, r, f, dtype=dtype, surface_mode=mode)
 
     def advance(self, st, clock, ledger, t_end, max_steps):
-        if self.mode == "full_swe":
-            return self.co.advance_to_time(st, clock, ledger, t_end, DURATION, max_steps=max_steps)
-        s, c, p, r, _, f = self.io
-        return orc.advance_to_time(st, clock, ledger, t_end, DURATION, s, c, p, r, f, max_steps=max_steps,
-                                   dtype=self.dtype, surface_mode=self.mode)
+        return self.co.advance_to_time(st, clock, ledger, t_end, DURATION, max_steps=max_steps)
 
 
 def oracle_300(out_dir: Path, threads: int, t_end=300.0, mode="full_swe"):
