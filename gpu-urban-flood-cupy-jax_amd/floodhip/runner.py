@@ -47,37 +47,95 @@ def run_hip_v1_parity(inputs: LoadedParityInputs, *, duration_s: float, max_step
     s = solver or HipParitySolver(inputs.surface, inputs.coupling, inputs.pipe, inputs.recharge,
                                   inputs.forcing, inputs.initial_state, boundary_stage_m=boundary_stage_m,
                                   **solver_opts)
-    area = np.asarray(inputs.surface.cell_area, np.float64)
-    initial_surface = float(np.sum(np.asarray(inputs.initial_state.h, np.float64) * area))
-    initial_node = float(np.sum(np.asarray(inputs.initial_state.node_volume, np.float64)))
     requested = output_targets(duration_s, snapshot_interval_s)
-    depth, speed, nodes, times, steps_at, chunk_rt = [], [], [], [], [], []
+    snaps = _Snapshots()
     run_started = perf_counter()
     ledger = None
-    st = None
     for i, req in enumerate(requested):
         target = np.float32(1.0e-6) if i == 0 else req
         t0 = perf_counter()
         clock, ledger = s.advance_to(float(target), float(duration_s), max_steps_per_hour,
                                      raise_on_budget=False)
-        chunk_rt.append(perf_counter() - t0)
+        snaps.chunk_rt.append(perf_counter() - t0)
+        snaps.take(s, clock, float(target), max_steps_per_hour)
+    out = _result(inputs, s, snaps, perf_counter() - run_started, ledger, duration_s, max_steps_per_hour,
+                  len(requested), keep_final_state)
+    if solver is None:
+        s.close()
+    return out
+
+
+def run_hip_ensemble(members, *, duration_s: float, max_steps_per_hour: int = 20_000,
+                     snapshot_interval_s: float = 3600.0, keep_final_state: bool = False,
+                     **solver_opts) -> list:
+    """The batched form beside ``run_hip_v1_parity``: every member (a
+    ``LoadedParityInputs``) advances in one ``HipEnsemble`` to the same output
+    times, each with its own CFL dt, clock, step budget and FP64 ledger.  Returns
+    one result per member, in ``run_hip_v1_parity``'s format (its ``runtime_s``
+    and chunk times are the batch's)."""
+    from .solver import HipEnsemble
+    if duration_s <= 0:
+        raise ConfigurationError("Parity duration must be positive.")
+    members = list(members)
+    requested = output_targets(duration_s, snapshot_interval_s)
+    snaps = [_Snapshots() for _ in members]
+    with HipEnsemble(members, **solver_opts) as ens:
+        run_started = perf_counter()
+        for i, req in enumerate(requested):
+            target = np.float32(1.0e-6) if i == 0 else req
+            t0 = perf_counter()
+            ens.advance_to(float(target), float(duration_s), max_steps_per_hour, raise_on_budget=False)
+            chunk = perf_counter() - t0
+            for m, sn in enumerate(snaps):
+                ens.select(m)
+                sn.chunk_rt.append(chunk)
+                sn.take(ens, ens.clock(), float(target), max_steps_per_hour)
+        runtime = perf_counter() - run_started
+        out = []
+        for m, (inp, sn) in enumerate(zip(members, snaps)):
+            ens.select(m)
+            out.append(_result(inp, ens, sn, runtime, ens.ledger(), duration_s, max_steps_per_hour,
+                               len(requested), keep_final_state))
+        ens.select(0)
+    return out
+
+
+class _Snapshots:
+    """Per-run snapshot lists: depth, speed, node volume, time and step count at
+    each output time, after the runner's checks (parity_runner.py:311-322)."""
+
+    def __init__(self):
+        self.depth, self.speed, self.nodes, self.times, self.steps_at, self.chunk_rt = [], [], [], [], [], []
+        self.state = None
+
+    def take(self, s, clock, target: float, max_steps_per_hour: int):
         reached = clock["time_s"]
-        if reached + 1.0e-3 < float(target):
+        if reached + 1.0e-3 < target:
             raise SimulationError(f"max_steps_per_hour={max_steps_per_hour} reached only {reached:.3f}s "
-                                  f"of target {float(target):.3f}s.")
+                                  f"of target {target:.3f}s.")
         st = s.get_state()
         if not np.isfinite(st.h).all():
             raise SimulationError(f"Non-finite surface depth at t={reached:.3f}s.")
         if not np.isfinite(st.node_volume).all():
             raise SimulationError(f"Non-finite drainage storage at t={reached:.3f}s.")
-        depth.append(st.h.astype(np.float32))
-        speed.append(_speed(st))
-        nodes.append(st.node_volume.astype(np.float32))
-        times.append(reached)
-        steps_at.append(int(clock["global_step"]))
-    runtime = perf_counter() - run_started
-    depth_a = np.stack(depth)
-    node_a = np.stack(nodes)
+        self.depth.append(st.h.astype(np.float32))
+        self.speed.append(_speed(st))
+        self.nodes.append(st.node_volume.astype(np.float32))
+        self.times.append(reached)
+        self.steps_at.append(int(clock["global_step"]))
+        self.state = st
+
+
+def _result(inputs, s, snaps: _Snapshots, runtime: float, ledger, duration_s: float, max_steps_per_hour: int,
+            n_targets: int, keep_final_state: bool) -> dict:
+    """The run's arrays and its report (s: the solver, with the run's member selected)."""
+    area = np.asarray(inputs.surface.cell_area, np.float64)
+    initial_surface = float(np.sum(np.asarray(inputs.initial_state.h, np.float64) * area))
+    initial_node = float(np.sum(np.asarray(inputs.initial_state.node_volume, np.float64)))
+    depth_a = np.stack(snaps.depth)
+    node_a = np.stack(snaps.nodes)
+    st = snaps.state
+    chunk_rt, steps_at, times = snaps.chunk_rt, snaps.steps_at, snaps.times
     led = {name: float(v) for name, v in zip(LEDGER_FIELDS, ledger)}
     final_surface = float(np.sum(depth_a[-1].astype(np.float64) * area))
     final_node = float(np.sum(node_a[-1].astype(np.float64)))
@@ -92,7 +150,7 @@ def run_hip_v1_parity(inputs: LoadedParityInputs, *, duration_s: float, max_step
     if "building_mask" in inputs.host:
         valid = valid & ~np.asarray(inputs.host["building_mask"], bool)
     mode = s.options.surface_mode
-    budget = int(len(requested) * max_steps_per_hour)
+    budget = int(n_targets * max_steps_per_hour)
     # the reference's report fields (parity_runner.py:346-388), with this backend's values
     report = {
         "implementation": "floodhip_mi355x", "model_version": "0.3.0-parity",
@@ -127,12 +185,10 @@ def run_hip_v1_parity(inputs: LoadedParityInputs, *, duration_s: float, max_step
         "observationally_validated": False, "input_report": inputs.report,
     }
     out = {"snapshot_times_s": np.asarray(times, np.float32), "steps_at_snapshot": np.asarray(steps_at, np.int64),
-           "depth_snapshots_m": depth_a, "speed_snapshots_mps": np.stack(speed),
+           "depth_snapshots_m": depth_a, "speed_snapshots_mps": np.stack(snaps.speed),
            "drain_node_volume_snapshots_m3": node_a, "report": report}
     if keep_final_state:
         out["final_state"] = st
-    if solver is None:
-        s.close()
     return out
 
 

@@ -127,3 +127,33 @@ def test_same_geometry_other_active_cells_is_not_shared():
             for name in ("h", "hu", "hv", "node_volume"):
                 assert np.array_equal(getattr(st, name), getattr(ref[m][0], name)), (share, m, name)
     assert footprint[True] == footprint[False], footprint
+
+
+def test_ensemble_runner_matches_plain_runner_per_member():
+    """runner.run_hip_ensemble (one batch to the output times) against
+    runner.run_hip_v1_parity on each member alone: the snapshots are bit-identical,
+    the step counts and ledgers equal, and the eight gates pass with zero error."""
+    from floodhip import gates
+    from floodhip.runner import run_hip_ensemble, run_hip_v1_parity
+    members = _members()
+    batch = run_hip_ensemble(members, duration_s=90.0, snapshot_interval_s=30.0, compensated=True)
+    assert len(batch) == len(members)
+    for inp, got in zip(members, batch):
+        ref = run_hip_v1_parity(inp, duration_s=90.0, snapshot_interval_s=30.0, compensated=True)
+        assert np.array_equal(got["snapshot_times_s"], ref["snapshot_times_s"])
+        assert np.array_equal(got["depth_snapshots_m"], ref["depth_snapshots_m"])
+        assert np.array_equal(got["drain_node_volume_snapshots_m3"], ref["drain_node_volume_snapshots_m3"])
+        assert got["report"]["step_count"] == ref["report"]["step_count"]
+        assert got["report"]["steps_at_snapshot"] == ref["report"]["steps_at_snapshot"]
+        for k in ("rainfall_input_m3", "infiltration_loss_m3", "outfall_discharge_m3", "surface_capture_m3"):
+            assert got["report"][k] == ref["report"][k], k
+        res = gates.validate(ref, got, inp.surface.cell_area)
+        # the gates' CSI is intersection / max(union, 1): 0 on a snapshot where
+        # neither run has a cell at 5 cm or more (the dry-start member's first
+        # snapshot), 1 on every other snapshot of identical runs
+        for hour in res["hourly"]:
+            wet = bool((ref["depth_snapshots_m"][hour["snapshot_index"]] >= gates.DEPTH_THRESHOLD_M).any())
+            assert hour["flood_extent_csi"] == (1.0 if wet else 0.0)
+        failed = {k for k, ok in res["gates"].items() if not ok}
+        assert failed <= {"all_hour_csi_ge_0_95"}, res["gates"]
+        assert res["summary"]["maximum_hourly_valid_depth_rmse_m"] == 0.0
