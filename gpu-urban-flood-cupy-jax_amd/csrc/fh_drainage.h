@@ -205,7 +205,7 @@ __device__ __forceinline__ real surcharge_request(const KP<real>& p, const typen
 // after a second barrier each inlet stores its net exchange (capture -
 // surcharge) in its node slot.  Dynamic LDS: two reals per inlet of the block
 // and one per cell.
-template <typename real>
+template <typename real, bool GATE>
 __global__ void __launch_bounds__(kInletThreads) k_inlet(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
         const typename V4<real>::type* __restrict__ state, double* __restrict__ part) {
     typedef typename V4<real>::type V;
@@ -214,6 +214,7 @@ __global__ void __launch_bounds__(kInletThreads) k_inlet(Ctl* __restrict__ ctls,
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
     const int t = threadIdx.x;
     const int mem = d.seg_icell.blk_member[b];
+    if (GATE && !ctls[mem].active) return;      // the whole block: one member
     const int mend = d.seg_icell.mem_end[mem];
     const int nent = d.ic_ptr[d.Ci];              // inlet entries of the batch
     const int e0 = d.ic_ptr[b * kInletRows], e1 = d.ic_ptr[b * kInletRows + kInletRows];
@@ -418,7 +419,7 @@ __device__ __forceinline__ real link_request(const KP<real>& p, const LinkData<r
 // 'from' end stores req[l].  Dynamic LDS: one real per item of the block.
 // The blocks after the node blocks (kLinkThreads inlet cells each) apply the
 // inlet cells' new depths.
-template <typename real>
+template <typename real, bool GATE>
 __global__ void __launch_bounds__(kLinkThreads) k_link_node(const Ctl* __restrict__ ctls, KP<real> p,
         DrainDev<real> d, int node_blocks, typename V4<real>::type* __restrict__ state) {
     constexpr int KB = kRowEntries;
@@ -430,6 +431,7 @@ __global__ void __launch_bounds__(kLinkThreads) k_link_node(const Ctl* __restric
         const int j = (bid - node_blocks) * (int)blockDim.x + (int)threadIdx.x;
         if (j >= d.Ci) return;
         const int mi = d.seg_icell.blk_member[j / kInletRows];
+        if (GATE && !ctls[mi].active) return;
         const int c = d.ic_cells[j];
         const real h = d.ic_h[j];
         FH_ISSUE_LOADS_OF("s"(state));
@@ -441,6 +443,7 @@ __global__ void __launch_bounds__(kLinkThreads) k_link_node(const Ctl* __restric
     const int b = bid;
     const int t = threadIdx.x;
     const int mem = d.seg_node.blk_member[b];
+    if (GATE && !ctls[mem].active) return;      // the whole block: one member
     const int mend = d.seg_node.mem_end[mem];
     const int N = d.N;
     const int it0 = d.lb_ptr[b], it1 = d.lb_ptr[b + 1];
@@ -519,13 +522,14 @@ __global__ void __launch_bounds__(kLinkThreads) k_link_node(const Ctl* __restric
 // cell fallback); writes the link flow for links this node is 'from' of.
 // A node whose fallback volume is the only return its surface cell receives
 // (nr_cell >= 0) applies it here, as k_return would: h + (0 + fb / A).
-template <typename real>
+template <typename real, bool GATE>
 __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
                                                     typename V4<real>::type* __restrict__ state,
                                                     double* __restrict__ part) {
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
     const int n = b * blockDim.x + threadIdx.x;
     const int mem = d.seg_node.blk_member[b];
+    if (GATE && !ctls[mem].active) return;
     const real zero = real(0);
     const int k0 = d.nl2_ptr[n], k1 = d.nl2_ptr[n + 1];
     const bool dry = d.dry != 0;
@@ -659,12 +663,13 @@ __global__ void __launch_bounds__(256) k_node_update(Ctl* __restrict__ ctls, KP<
 // Return pass (one thread per receiving cell of rx_rows): weighted entry
 // returns, node fallback returns, then recharge capture/loss
 // (v1_parity.py:371-404, 415-427).
-template <typename real>
+template <typename real, bool GATE>
 __global__ void __launch_bounds__(256) k_return(Ctl* __restrict__ ctls, KP<real> p, DrainDev<real> d,
         typename V4<real>::type* __restrict__ state, const real* __restrict__ area, double* __restrict__ part) {
     const int b = xcd_block(blockIdx.x, gridDim.x, p.remap);
     const int x = b * blockDim.x + threadIdx.x;
     const int mem = d.seg_rx.blk_member[b];
+    if (GATE && !ctls[mem].active) return;
     const int j = d.rx_rows[x];
     const real zero = real(0);
     const int c = d.rc_cells[j];

@@ -235,6 +235,18 @@ struct StepPlan {
     RainCache rain;          // rain cache after the lookup at t
 };
 
+// Whether the member's step executes, as plan_step decides it.  The kernels of a
+// batch (GATE: the context holds more than one member) test it, or the clock's copy of it, before their first load:
+// a member that has reached its output time while others step on then costs
+// its blocks' dispatch, not its bytes.  Every member of a batch stops at its own
+// time, and in the members' last steps most blocks belong to idle members.  The
+// plain solver loads ahead of the test instead, so its gathers start one memory
+// round trip earlier.
+__device__ __forceinline__ bool step_runs(const Ctl* ctl, int parity) {
+    const Clock& c = ctl->clk[parity];
+    return (c.t < ctl->target) && (c.interval_step < ctl->max_steps);
+}
+
 template <typename real>
 __device__ __forceinline__ StepPlan<real> plan_step(const Ctl* ctl, const Clock& c, const KP<real>& p,
                                                     const ForcingTables& ft, int m, double gmin) {
@@ -385,7 +397,7 @@ struct EdgeSeg {
 // by the member-local edge.  stage_h: the stage ghost depth of each edge position
 // (read at stage edges only).
 // ---------------------------------------------------------------------------
-template <typename real, int EPT, bool SH>
+template <typename real, int EPT, bool SH, bool GATE>
 __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
         const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
@@ -419,6 +431,7 @@ __global__ void __launch_bounds__(256) k_flux(Ctl* __restrict__ ctls, KP<real> p
         b = i;
         m = es.seg.blk_member[b];
     }
+    if (GATE && !ctls[m].clk[parity].active) return;
     // every load ahead of the control block: the grid covers the padded edge
     // space exactly, padding edges point at cell 0 as copy ghosts
     int lft[EPT], rgt[EPT];
@@ -523,7 +536,7 @@ __global__ void __launch_bounds__(256) k_li_cfl(Ctl* __restrict__ ctls, KP<real>
 // (codes 0, 2, 3, inactive neighbours).  Writes rec[e] = {Q, Q nx, Q ny, 0} for
 // the update and qv[e] = Q, which the donor pass gathers (4 bytes, not 16).
 // SH as in k_flux.
-template <typename real, bool SH>
+template <typename real, bool SH, bool GATE>
 __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
         ForcingTables ft, const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
@@ -540,6 +553,7 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
         b = i;
         m = es.seg.blk_member[b];
     }
+    if (GATE && !step_runs(ctls + m, parity)) return;
     const int e = b * (int)blockDim.x + threadIdx.x;
     const int g = e - g0;
     const int l = el[g] + cb, r0 = er[g];
@@ -605,7 +619,7 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
 // interior edge has exactly one donor) into T[edge].  SH: the cell tables (ce*,
 // perm) are shared and indexed by the member-local cell; their edge positions
 // are member-local.
-template <typename real, bool SH>
+template <typename real, bool SH, bool GATE>
 __global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
         ForcingTables ft, const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
         const unsigned char* __restrict__ perm, const real* __restrict__ qv,
@@ -620,6 +634,7 @@ __global__ void __launch_bounds__(256) k_li_donor(Ctl* __restrict__ ctls, KP<rea
         b = i;
         m = seg.blk_member[b];
     }
+    if (GATE && !step_runs(ctls + m, parity)) return;
     const int c = b * blockDim.x + threadIdx.x;
     const int cl = c - c0;
     const int ent[3] = {ce0[cl], ce1[cl], ce2[cl]};
@@ -697,7 +712,7 @@ struct CellSources {
 // the flux divergence and friction: rec holds {Q, Q nx, Q ny} per edge and recR
 // the donor-limited transfer T of each interior edge (k_li_flux, k_li_donor).
 // SH: the cell->edge table is shared (as in k_li_donor).
-template <typename real, bool COMP, bool CLS, bool LI, bool SH>
+template <typename real, bool COMP, bool CLS, bool LI, bool SH, bool GATE>
 __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real> p, Seg seg, int parity,
         ForcingTables ft,
         const int* __restrict__ ce0, const int* __restrict__ ce1, const int* __restrict__ ce2,
@@ -716,6 +731,8 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
         b = i;
         m = seg.blk_member[b];
     }
+    // (the member's first block goes on: it carries the clock to the next step's slot)
+    if (GATE && !step_runs(ctls + m, parity) && b != seg.mem_blk0[m]) return;
     const int c = b * blockDim.x + threadIdx.x;
     const real zero = real(0), eps = p.eps;
     // every load of the cell (the grid covers the padded cell space exactly;

@@ -1646,27 +1646,28 @@ public:
     void launch_li_flux() {
         if (edges.blocks == 0) return;
         fh::EdgeSeg es{edge_seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
-#define FH_LIFLUX(SH_)                                                                                     \
-        go(fh::k_li_flux<real, SH_>, edges.blocks, bs, ctl, kp, es, par, forcing_tables(), d_el.p, d_er.p,   \
+#define FH_LIFLUX(SH_, G_)                                                                                 \
+        go(fh::k_li_flux<real, SH_, G_>, edges.blocks, bs, ctl, kp, es, par, forcing_tables(), d_el.p, d_er.p,   \
            d_nx.p, d_ny.p, d_len.p, d_state.p, d_lc.p, d_cell_n.p, d_rec.p, d_qli.p)
-        if (shared) FH_LIFLUX(true); else FH_LIFLUX(false);
+        if (shared) FH_LIFLUX(true, true); else if (M > 1) FH_LIFLUX(false, true); else FH_LIFLUX(false, false);
 #undef FH_LIFLUX
     }
     void launch_li_donor() {
-#define FH_DONOR(SH_)                                                                                      \
-        go(fh::k_li_donor<real, SH_>, cells.blocks, bs, ctl, kp, cell_seg(), par, forcing_tables(), d_ce0.p, \
+#define FH_DONOR(SH_, G_)                                                                                  \
+        go(fh::k_li_donor<real, SH_, G_>, cells.blocks, bs, ctl, kp, cell_seg(), par, forcing_tables(), d_ce0.p, \
            d_ce1.p, d_ce2.p, d_perm.p, d_qli.p, d_recR.p, d_state.p, d_area.p)
-        if (shared) FH_DONOR(true); else FH_DONOR(false);
+        if (shared) FH_DONOR(true, true); else if (M > 1) FH_DONOR(false, true); else FH_DONOR(false, false);
 #undef FH_DONOR
     }
     void launch_flux() {
         if (edges.blocks + orphans.blocks == 0) return;
         fh::EdgeSeg es{edge_seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
         const unsigned grid = (unsigned)(edges.blocks + orphans.blocks);
-#define FH_FLUX(EPT_, SH_)                                                                                    \
-        go(fh::k_flux<real, EPT_, SH_>, grid, bs, ctl, kp, es, par, d_el.p, d_er.p, d_nx.p, d_ny.p, d_len.p, \
+#define FH_FLUX(EPT_, SH_, G_)                                                                                \
+        go(fh::k_flux<real, EPT_, SH_, G_>, grid, bs, ctl, kp, es, par, d_el.p, d_er.p, d_nx.p, d_ny.p, d_len.p, \
            d_state.p, d_lc.p, d_rec.p, d_recR.p, d_stage_h.p)
-#define FH_FLUX_S(EPT_) if (shared) FH_FLUX(EPT_, true); else FH_FLUX(EPT_, false)
+#define FH_FLUX_S(EPT_) \
+        if (shared) FH_FLUX(EPT_, true, true); else if (M > 1) FH_FLUX(EPT_, false, true); else FH_FLUX(EPT_, false, false)
         if (ept == 4) { FH_FLUX_S(4); }
         else if (ept == 2) { FH_FLUX_S(2); }
         else { FH_FLUX_S(1); }
@@ -1676,12 +1677,14 @@ public:
     void launch_update() {
         const fh::CellSources<real> src{d_n2.p, d_rainm.p, d_f0.p, d_fc.p, d_k.p, d_cls.p, d_cls_tab.p, d_cls_hk.p};
         const fh::Seg cs = cell_seg();
-#define FH_UPDATE(COMP_, CLS_, LI_, SH_)                                                                    \
-        go(fh::k_update<real, COMP_, CLS_, LI_, SH_>, cells.blocks, bs, ctl, kp, cs, par,                        \
+#define FH_UPDATE(COMP_, CLS_, LI_, SH_, G_)                                                                \
+        go(fh::k_update<real, COMP_, CLS_, LI_, SH_, G_>, cells.blocks, bs, ctl, kp, cs, par,                        \
            forcing_tables(), d_ce0.p, d_ce1.p, d_ce2.p,                                                         \
            d_rec.p, d_recR.p, d_state.p, d_comp.p, d_wet.p, d_area.p, d_lc.p, src, d_part_upd.p)
 #define FH_UPDATE_S(COMP_, CLS_, LI_) \
-        if (shared) { FH_UPDATE(COMP_, CLS_, LI_, true); } else { FH_UPDATE(COMP_, CLS_, LI_, false); }
+        if (shared) { FH_UPDATE(COMP_, CLS_, LI_, true, true); }                                             \
+        else if (M > 1) { FH_UPDATE(COMP_, CLS_, LI_, false, true); }                                        \
+        else { FH_UPDATE(COMP_, CLS_, LI_, false, false); }
 #define FH_UPDATE_M(COMP_, CLS_) \
         if (li) { FH_UPDATE_S(COMP_, CLS_, true); } else { FH_UPDATE_S(COMP_, CLS_, false); }
         if (P.compensated) {
@@ -1706,19 +1709,30 @@ public:
         if (nodes.blocks == 0) return;
         if (icl.blocks > 0)
             timed("inlet", (double)d.Ci * (12 + 4 * rb) + (double)d.I * (8 + 5 * rb), [&] {
-                go_lds(fh::k_inlet<real>, icl.blocks, fh::kInletThreads, inlet_lds, ctl, kp, d, d_state.p,
-                       d_part_inlet.p);
+                if (M > 1)
+                    go_lds(fh::k_inlet<real, true>, icl.blocks, fh::kInletThreads, inlet_lds, ctl, kp, d, d_state.p,
+                           d_part_inlet.p);
+                else
+                    go_lds(fh::k_inlet<real, false>, icl.blocks, fh::kInletThreads, inlet_lds, ctl, kp, d, d_state.p,
+                           d_part_inlet.p);
             });
         timed("link_node", (double)d.N * (8 + 4 * rb) + (double)d.L * (8 + 11 * rb), [&] {
-            go_lds(fh::k_link_node<real>, nodes.blocks + (icl.total + fh::kLinkThreads - 1) / fh::kLinkThreads,
-                   fh::kLinkThreads, link_node_lds, ctl, kp, d, nodes.blocks, d_state.p);
+            const unsigned grid = nodes.blocks + (icl.total + fh::kLinkThreads - 1) / fh::kLinkThreads;
+            if (M > 1)
+                go_lds(fh::k_link_node<real, true>, grid, fh::kLinkThreads, link_node_lds, ctl, kp, d, nodes.blocks,
+                       d_state.p);
+            else
+                go_lds(fh::k_link_node<real, false>, grid, fh::kLinkThreads, link_node_lds, ctl, kp, d, nodes.blocks,
+                       d_state.p);
         });
         timed("node_update", (double)d.N * (16 + 14 * rb) + (double)d.L * (16 + 3 * rb), [&] {
-            go(fh::k_node_update<real>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
+            if (M > 1) go(fh::k_node_update<real, true>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
+            else go(fh::k_node_update<real, false>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
         });
         if (rxl.blocks > 0)
             timed("return", (double)d.Cr * (16 + 3 * rb) + (double)d.N * 2 * rb, [&] {
-                go(fh::k_return<real>, rxl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_ret.p);
+                if (M > 1) go(fh::k_return<real, true>, rxl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_ret.p);
+                else go(fh::k_return<real, false>, rxl.blocks, 256, ctl, kp, d, d_state.p, d_area.p, d_part_ret.p);
             });
     }
 
@@ -2018,9 +2032,9 @@ public:
             d.dry = 1;
             d.dry_cap = cap ? 1 : 0;
             d.dry_tw = tw ? d_dw_tw.p : nullptr;
-            go_lds(fh::k_link_node<real>, nodes.blocks, fh::kLinkThreads, link_node_lds, ctl, kp, d, nodes.blocks,
+            go_lds(fh::k_link_node<real, false>, nodes.blocks, fh::kLinkThreads, link_node_lds, ctl, kp, d, nodes.blocks,
                    d_state.p);
-            go(fh::k_node_update<real>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
+            go(fh::k_node_update<real, false>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
         }
         fold();
         pull_ctl();
