@@ -242,8 +242,16 @@ void locality_order(Member<real>& m) {
         mark[(size_t)root] = place ? -1 : stamp;
         for (size_t q = 0; q < queue.size(); ++q) {
             const int c = queue[q];
+            // neighbours by ascending degree, ties in adjacency order (a stable
+            // insertion sort: a cell has a few neighbours, and std::stable_sort
+            // would allocate a buffer per cell)
             nb.assign(adj.begin() + deg[(size_t)c], adj.begin() + deg[(size_t)c + 1]);
-            std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return degree(a) < degree(b); });
+            for (size_t i = 1; i < nb.size(); ++i) {
+                const int v = nb[i], dv = degree(v);
+                size_t j = i;
+                for (; j > 0 && degree(nb[j - 1]) > dv; --j) nb[j] = nb[j - 1];
+                nb[j] = v;
+            }
             for (int x : nb) {
                 const int mk = mark[(size_t)x];
                 if (mk == -1 || mk == (place ? -1 : stamp)) continue;
@@ -276,22 +284,28 @@ void locality_order(Member<real>& m) {
 // keep their order at the end) and links their lower end node, so the node- and
 // link-side gathers of the drainage kernels walk the arrays in the same order
 // as the cells they serve.
+// Positions pos[i] (offset by base) of the items i = begin .. end-1 ordered by
+// key(i) in [0, K), ties in index order: a counting sort, the order a stable
+// comparison sort by key gives, in linear time.
+template <class Key>
+void positions_by_key(int64_t begin, int64_t end, int64_t K, int64_t base, Key key, std::vector<int>& pos) {
+    std::vector<int64_t> start((size_t)K + 1, 0);
+    for (int64_t i = begin; i < end; ++i) ++start[(size_t)key(i) + 1];
+    for (int64_t k = 0; k < K; ++k) start[(size_t)k + 1] += start[(size_t)k];
+    for (int64_t i = begin; i < end; ++i) pos[(size_t)i] = (int)(base + start[(size_t)key(i)]++);
+}
+
 template <typename real>
 void assign_positions(Member<real>& m) {
     const int64_t C = m.C, Ei = m.E_int, E = m.E;
     m.cpos.assign((size_t)C, 0);
     for (int64_t i = 0; i < C; ++i) m.cpos[(size_t)m.bfs[(size_t)i]] = (int)i;
     // edges: by the lower new cell index (interior), by the new left cell (ghost)
-    std::vector<int64_t> idx((size_t)E);
-    std::iota(idx.begin(), idx.end(), 0);
-    auto key = [&](int64_t e) {
-        const int l = m.cpos[(size_t)m.el[e]];
-        return e < Ei ? std::min(l, m.cpos[(size_t)m.er[e]]) : l;
-    };
-    std::stable_sort(idx.begin(), idx.begin() + Ei, [&](int64_t a, int64_t b) { return key(a) < key(b); });
-    std::stable_sort(idx.begin() + Ei, idx.end(), [&](int64_t a, int64_t b) { return key(a) < key(b); });
     m.epos.assign((size_t)E, 0);
-    for (int64_t i = 0; i < E; ++i) m.epos[(size_t)idx[(size_t)i]] = (int)i;
+    positions_by_key(0, Ei, C, 0, [&](int64_t e) {
+        return std::min(m.cpos[(size_t)m.el[e]], m.cpos[(size_t)m.er[e]]);
+    }, m.epos);
+    positions_by_key(Ei, E, C, Ei, [&](int64_t e) { return m.cpos[(size_t)m.el[e]]; }, m.epos);
     // nodes and links
     const int64_t N = m.N, L = m.L;
     std::vector<int64_t> nkey((size_t)N);
@@ -300,17 +314,12 @@ void assign_positions(Member<real>& m) {
         const int n = m.inode[k];
         nkey[(size_t)n] = std::min<int64_t>(nkey[(size_t)n], m.cpos[(size_t)m.icell[k]]);
     }
-    std::vector<int64_t> nidx((size_t)N);
-    std::iota(nidx.begin(), nidx.end(), 0);
-    std::stable_sort(nidx.begin(), nidx.end(), [&](int64_t a, int64_t b) { return nkey[(size_t)a] < nkey[(size_t)b]; });
     m.npos.assign((size_t)N, 0);
-    for (int64_t i = 0; i < N; ++i) m.npos[(size_t)nidx[(size_t)i]] = (int)i;
-    std::vector<int64_t> lidx((size_t)L);
-    std::iota(lidx.begin(), lidx.end(), 0);
-    auto lkey = [&](int64_t l) { return std::min(m.npos[(size_t)m.lfrom[l]], m.npos[(size_t)m.lto[l]]); };
-    std::stable_sort(lidx.begin(), lidx.end(), [&](int64_t a, int64_t b) { return lkey(a) < lkey(b); });
+    positions_by_key(0, N, C + N, 0, [&](int64_t n) { return nkey[(size_t)n]; }, m.npos);
     m.lpos.assign((size_t)L, 0);
-    for (int64_t i = 0; i < L; ++i) m.lpos[(size_t)lidx[(size_t)i]] = (int)i;
+    positions_by_key(0, L, std::max<int64_t>(N, 1), 0, [&](int64_t l) {
+        return std::min(m.npos[(size_t)m.lfrom[l]], m.npos[(size_t)m.lto[l]]);
+    }, m.lpos);
 }
 
 class EngineBase {
