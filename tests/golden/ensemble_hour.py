@@ -19,6 +19,9 @@ members run one after another (advance calls only).
 
     python tests/golden/ensemble_hour.py config5 OUT.json [--options 256,2,0]
     python tests/golden/ensemble_hour.py config4 OUT.json [--options 256,1,2]
+
+--duration S runs S simulated seconds instead of the hour; --batch-only skips the
+plain runs and the comparison (for a kernel trace of the batch alone).
 """
 
 from __future__ import annotations
@@ -59,6 +62,8 @@ def main():
     ap.add_argument("config", choices=["config5", "config4"])
     ap.add_argument("out", type=Path)
     ap.add_argument("--options", default=None, help="block size, edges per thread, pack order of the batch")
+    ap.add_argument("--duration", type=float, default=DURATION, help="simulated seconds")
+    ap.add_argument("--batch-only", action="store_true", help="run the batch alone")
     a = ap.parse_args()
     t0 = time.perf_counter()
     members = members_of(a.config)
@@ -67,14 +72,19 @@ def main():
     if a.options:
         bs, ept, order = (int(v) for v in a.options.split(","))
         opts.update(block_size=bs, edges_per_thread=ept, pack_order=order)
-    batch = run_hip_ensemble(members, duration_s=DURATION, max_steps_per_hour=BUDGET, **opts)
+    batch = run_hip_ensemble(members, duration_s=a.duration, max_steps_per_hour=BUDGET, **opts)
     batch_s = float(sum(batch[0]["report"]["chunk_runtimes_s"]))
     print(f"batch: {batch_s:.3f}s of advance calls, steps "
           f"{min(r['report']['step_count'] for r in batch)}-{max(r['report']['step_count'] for r in batch)}",
           flush=True)
+    if a.batch_only:
+        a.out.write_text(json.dumps({"config": a.config, "duration_s": a.duration, "members": len(members),
+                                     "batch_advance_seconds": batch_s,
+                                     "steps": [int(r["report"]["step_count"]) for r in batch]}) + "\n")
+        return
     rows, seq_s, tri_steps = [], 0.0, 0
     for m, (inp, got) in enumerate(zip(members, batch)):
-        ref = run_hip_v1_parity(inp, duration_s=DURATION, max_steps_per_hour=BUDGET, keep_final_state=False,
+        ref = run_hip_v1_parity(inp, duration_s=a.duration, max_steps_per_hour=BUDGET, keep_final_state=False,
                                 compensated=True, use_graph=True)
         seq_s += float(sum(ref["report"]["chunk_runtimes_s"]))
         cells = int(inp.surface.cell_area.shape[0])
@@ -92,7 +102,7 @@ def main():
         rows.append(row)
         print(f"member {m}: {cells} triangles, {row['steps']} steps, bit-identical {row['snapshots_bit_identical']}, "
               f"gates {row['gates_passed']}", flush=True)
-    out = {"config": a.config, "duration_s": DURATION, "members": len(members), "options": opts,
+    out = {"config": a.config, "duration_s": a.duration, "members": len(members), "options": opts,
            "triangles": int(sum(r["triangles"] for r in rows)),
            "batch_advance_seconds": batch_s, "sequential_plain_advance_seconds": seq_s,
            "batch_triangle_steps_per_s": tri_steps / batch_s, "sequential_triangle_steps_per_s": tri_steps / seq_s,
