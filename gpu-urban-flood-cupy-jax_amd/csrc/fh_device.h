@@ -141,6 +141,12 @@ struct KP {                 // kernel constants, pre-rounded to the state precis
     unsigned long long* cfl_slots;   // the CFL minimum slots (cfl_slot)
 };
 
+// A local-inertial edge record {Q, Q nx, Q ny}: three values, no padding.
+template <typename real>
+struct Rec3 {
+    real x, y, z;
+};
+
 // NaN-propagating max/min, like numpy.maximum / jax.numpy.maximum (fmax/fmin
 // would drop a NaN operand and hide a blown-up state from the finite checks).
 // In float they are gfx950's v_maximum3_f32 / v_minimum3_f32 (IEEE 754-2019

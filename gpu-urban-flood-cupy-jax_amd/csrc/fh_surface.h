@@ -533,15 +533,16 @@ __global__ void __launch_bounds__(256) k_li_cfl(Ctl* __restrict__ ctls, KP<real>
 // local_inertial.py:20-48): the LISFLOOD-FP update of the face discharge from
 // the water-surface slope between the two cells, friction semi-implicit;
 // outflow-only at routable boundary edges (codes 4 and 1), zero at walls
-// (codes 0, 2, 3, inactive neighbours).  Writes rec[e] = {Q, Q nx, Q ny, 0} for
-// the update and qv[e] = Q, which the donor pass gathers (4 bytes, not 16).
+// (codes 0, 2, 3, inactive neighbours).  Writes rec[e] = {Q, Q nx, Q ny} (12
+// bytes: the update reads no fourth value) for the update and qv[e] = Q, which
+// the donor pass gathers (4 bytes, not 12).
 // SH as in k_flux.
 template <typename real, bool SH, bool GATE>
 __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real> p, EdgeSeg es, int parity,
         ForcingTables ft, const int* __restrict__ el, const int* __restrict__ er, const real* __restrict__ enx,
         const real* __restrict__ eny, const real* __restrict__ elen,
         const typename V4<real>::type* __restrict__ state, const real* __restrict__ lc,
-        const real* __restrict__ man, typename V4<real>::type* __restrict__ rec, real* __restrict__ qv) {
+        const real* __restrict__ man, Rec3<real>* __restrict__ rec, real* __restrict__ qv) {
     typedef typename V4<real>::type V;
     const int i = xcd_block(blockIdx.x, es.edge_blocks, p.remap);
     int b, m, g0 = 0, cb = 0;
@@ -604,9 +605,7 @@ __global__ void __launch_bounds__(256) k_li_flux(Ctl* __restrict__ ctls, KP<real
         q = Divisor<real>(denominator).div(numerator);
         if (!interior) q = vmax(q, zero);
     }
-    V o;
-    o.x = q; o.y = q * nx; o.z = q * ny; o.w = zero;
-    rec[e] = o;
+    rec[e] = Rec3<real>{q, q * nx, q * ny};
     qv[e] = q;
 }
 
@@ -749,7 +748,13 @@ __global__ void __launch_bounds__(256) k_update(Ctl* __restrict__ ctls, KP<real>
     for (int k = 0; k < 3; ++k) {
         const int en = ent[k];
         const int pos = en >= 0 ? (en >> 2) + eb : 0;
-        rr[k] = rec[pos];
+        if constexpr (LI) {
+            // the local-inertial records are {Q, Q nx, Q ny}, 12 bytes (k_li_flux)
+            const Rec3<real> t = reinterpret_cast<const Rec3<real>*>(rec)[pos];
+            rr[k].x = t.x; rr[k].y = t.y; rr[k].z = t.z; rr[k].w = real(0);
+        } else {
+            rr[k] = rec[pos];
+        }
         rR[k] = recR[pos];
     }
     const real A = area_[c];

@@ -1657,7 +1657,8 @@ public:
         fh::EdgeSeg es{edge_seg(), edges.blocks, seg_orph.seg(), d_ocells.p};
 #define FH_LIFLUX(SH_, G_)                                                                                 \
         go(fh::k_li_flux<real, SH_, G_>, edges.blocks, bs, ctl, kp, es, par, forcing_tables(), d_el.p, d_er.p,   \
-           d_nx.p, d_ny.p, d_len.p, d_state.p, d_lc.p, d_cell_n.p, d_rec.p, d_qli.p)
+           d_nx.p, d_ny.p, d_len.p, d_state.p, d_lc.p, d_cell_n.p, reinterpret_cast<fh::Rec3<real>*>(d_rec.p), \
+           d_qli.p)
         if (shared) FH_LIFLUX(true, true); else if (M > 1) FH_LIFLUX(false, true); else FH_LIFLUX(false, false);
 #undef FH_LIFLUX
     }
