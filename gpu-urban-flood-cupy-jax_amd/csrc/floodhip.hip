@@ -2101,14 +2101,17 @@ public:
         for (const auto& m : mem) { Ct += (double)m.C; Et += (double)m.E; }
         const double R4 = sizeof(real), I4 = 4.0;
         // bytes each kernel must move (each required element once per step)
+        // (local-inertial: the edge records are {Q, Q nx, Q ny} plus Q and T apart,
+        // and the discharge gathers Manning n and the CFL length of both cells)
         const double flux_bytes = Et * (2 * I4 + 3 * R4) + Ct * 4 * R4 + Et * 5 * R4;
-        const double upd_bytes = Et * 5 * R4 + Ct * (3 * I4 + 8 * R4 + 2 * R4 + 4 * R4 + 2 * R4) +
+        const double li_flux_bytes = Et * (2 * I4 + 3 * R4) + Ct * 6 * R4 + Et * 4 * R4;
+        const double upd_bytes = Et * (li ? 4 : 5) * R4 + Ct * (3 * I4 + 8 * R4 + 2 * R4 + 4 * R4 + 2 * R4) +
                                  (P.compensated ? Ct * 8 * R4 : 0.0);
         fh::k_set_target<<<(M + 255) / 256, 256, 0, stream>>>(ctl, M, par, INFINITY, INFINITY, (int64_t)steps + 8);
         for (int i = 0; i < steps; ++i) {
             if (li) {
                 timed("li_cfl", 0.0, [&] { launch_li_cfl(); });
-                timed("li_flux", flux_bytes, [&] { launch_li_flux(); });
+                timed("li_flux", li_flux_bytes, [&] { launch_li_flux(); });
                 timed("li_donor", Ct * (3 * I4 + 1 + 2 * R4) + Et * 2 * R4, [&] { launch_li_donor(); });
             } else {
                 timed("flux", flux_bytes, [&] { launch_flux(); });
