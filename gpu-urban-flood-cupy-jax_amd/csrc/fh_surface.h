@@ -236,12 +236,13 @@ struct StepPlan {
 };
 
 // Whether the member's step executes, as plan_step decides it.  The kernels of a
-// batch (GATE: the context holds more than one member) test it, or the clock's copy of it, before their first load:
-// a member that has reached its output time while others step on then costs
-// its blocks' dispatch, not its bytes.  Every member of a batch stops at its own
-// time, and in the members' last steps most blocks belong to idle members.  The
-// plain solver loads ahead of the test instead, so its gathers start one memory
-// round trip earlier.
+// batch (GATE: the context holds more than one member) test it, or the clock's
+// copy of it, before their first load: a member that has reached its output
+// time while others step on then costs its blocks' dispatch, not its bytes.
+// Every member of a batch stops at its own time, and in the members' last steps
+// most blocks belong to idle members.  The plain solver's kernels (GATE false)
+// load ahead of the test instead, so their gathers start one memory round trip
+// earlier.
 __device__ __forceinline__ bool step_runs(const Ctl* ctl, int parity) {
     const Clock& c = ctl->clk[parity];
     return (c.t < ctl->target) && (c.interval_step < ctl->max_steps);
