@@ -253,7 +253,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--compensated", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
-    ap.add_argument("--block-size", type=int, default=256)
+    ap.add_argument("--block-size", type=int, default=64)
     ap.add_argument("--no-remap", action="store_true")
     ap.add_argument("--batch-members", type=int, default=16)
     ap.add_argument("--batch-steps", type=int, default=200)

@@ -2447,7 +2447,7 @@ void fh_default_params(fh_params* p) {
     p->precision = 32;
     p->use_graph = 1;
     p->steps_per_launch = 32;
-    p->block_size = 256;
+    p->block_size = 64;
     p->xcd_remap = 1;
     p->device = 0;
     p->edges_per_thread = 1;

@@ -21,7 +21,7 @@ def _rate(ens, sizes, steps, horizon):
 
 def candidates(uneven: bool):
     orders = (0, 1, 2) if uneven else (0,)
-    for bs, ept, order in itertools.product((128, 256), (1, 2), orders):
+    for bs, ept, order in itertools.product((64, 128, 256), (1, 2), orders):
         yield dict(block_size=bs, edges_per_thread=ept, pack_order=order)
 
 

@@ -74,7 +74,7 @@ class SolverOptions(NamedTuple):
     fixed_dt: float | None = None
     use_graph: bool = True
     steps_per_launch: int = 32
-    block_size: int = 256
+    block_size: int = 64
     xcd_remap: bool = True
     device: int = 0
     edges_per_thread: int = 1

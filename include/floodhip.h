@@ -62,7 +62,7 @@ typedef struct {
     int32_t precision;          /* 32 (float state) or 64 (double state: FP64 oracle runs) */
     int32_t use_graph;          /* 1: replay captured hipGraphs of steps_per_launch steps */
     int32_t steps_per_launch;   /* steps per graph / per host check (default 32) */
-    int32_t block_size;         /* threads per block for the edge and cell kernels (64..256) */
+    int32_t block_size;         /* threads per block for the edge and cell kernels (64..256; default 64, one wavefront) */
     int32_t xcd_remap;          /* 1: map blocks so each XCD works a contiguous slice */
     int32_t device;             /* HIP device ordinal */
     int32_t edges_per_thread;   /* 1, 2 or 4 edges per thread in the flux kernel */
