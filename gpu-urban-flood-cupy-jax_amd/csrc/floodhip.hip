@@ -929,7 +929,7 @@ public:
             for (int64_t j = 0; j < s.Rcv; ++j) mark[s.cp(s.rcv[j])] = 1;
             for (int64_t c = 0; c < s.C; ++c) if (mark[c]) xcells[i].push_back((int)c);
         }
-        icl.build(counts_of(icells), 256);
+        icl.build(counts_of(icells), fh::kInletRows);
         rcl.build(counts_of(rcells), 256);
         xcl.build(counts_of(xcells), 256);
         // k_cfl / k_li_cfl blocks: one per 2,048 of the member's cells (at most 512),
