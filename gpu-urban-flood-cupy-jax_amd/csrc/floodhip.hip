@@ -909,7 +909,7 @@ public:
         };
         cells.build(counts([](const Member<real>& s) { return s.C; }), bs);
         edges.build(counts([](const Member<real>& s) { return s.E; }), bs * ept);
-        nodes.build(counts([](const Member<real>& s) { return s.N; }), 256);
+        nodes.build(counts([](const Member<real>& s) { return s.N; }), fh::kNodeBlock);
         links.build(counts([](const Member<real>& s) { return s.L; }), 256);
         // per-member derived lists (inlet cells, return cells, receiver cells)
         std::vector<std::vector<int>> icells(M), rcells(M), xcells(M);
@@ -1736,8 +1736,8 @@ public:
                        d_state.p);
         });
         timed("node_update", (double)d.N * (16 + 14 * rb) + (double)d.L * (16 + 3 * rb), [&] {
-            if (M > 1) go(fh::k_node_update<real, true>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
-            else go(fh::k_node_update<real, false>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
+            if (M > 1) go(fh::k_node_update<real, true>, nodes.blocks, fh::kNodeBlock, ctl, kp, d, d_state.p, d_part_node.p);
+            else go(fh::k_node_update<real, false>, nodes.blocks, fh::kNodeBlock, ctl, kp, d, d_state.p, d_part_node.p);
         });
         if (rxl.blocks > 0)
             timed("return", (double)d.Cr * (16 + 3 * rb) + (double)d.N * 2 * rb, [&] {
@@ -2044,7 +2044,7 @@ public:
             d.dry_tw = tw ? d_dw_tw.p : nullptr;
             go_lds(fh::k_link_node<real, false>, nodes.blocks, fh::kLinkThreads, link_node_lds, ctl, kp, d, nodes.blocks,
                    d_state.p);
-            go(fh::k_node_update<real, false>, nodes.blocks, 256, ctl, kp, d, d_state.p, d_part_node.p);
+            go(fh::k_node_update<real, false>, nodes.blocks, fh::kNodeBlock, ctl, kp, d, d_state.p, d_part_node.p);
         }
         fold();
         pull_ctl();
