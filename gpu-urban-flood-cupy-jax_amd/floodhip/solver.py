@@ -68,6 +68,9 @@ class _Args:
         return a.ctypes.data_as(C.POINTER(C.c_uint8))
 
 
+BATCH_BLOCK_SIZE = 256   # HipEnsemble's default block size (plain contexts: 64, one wavefront)
+
+
 class SolverOptions(NamedTuple):
     precision: int = 32
     compensated: bool = False
@@ -431,6 +434,11 @@ class HipEnsemble(HipParitySolver):
 
     def __init__(self, members, options: SolverOptions | None = None, **kw):
         members = list(members)
+        if options is None and len(members) > 1:
+            # blocks of 256 threads unless given: a batch's members stop at their own
+            # times, and every idle member's blocks are still dispatched, so fewer,
+            # larger blocks cost less there (the autotuner may pick otherwise)
+            kw.setdefault("block_size", BATCH_BLOCK_SIZE)
         super().__init__(None, options=options, members=len(members), **kw)
         for m, inp in enumerate(members):
             self.load_member(m, inp.surface, inp.coupling, inp.pipe, inp.recharge, inp.forcing,

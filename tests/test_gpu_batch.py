@@ -135,11 +135,15 @@ def test_ensemble_runner_matches_plain_runner_per_member():
     the step counts and ledgers equal, and the eight gates pass with zero error."""
     from floodhip import gates
     from floodhip.runner import run_hip_ensemble, run_hip_v1_parity
+    from floodhip.solver import BATCH_BLOCK_SIZE
     members = _members()
     batch = run_hip_ensemble(members, duration_s=90.0, snapshot_interval_s=30.0, compensated=True)
     assert len(batch) == len(members)
     for inp, got in zip(members, batch):
-        ref = run_hip_v1_parity(inp, duration_s=90.0, snapshot_interval_s=30.0, compensated=True)
+        # the batch's default launch shape, so that the FP64 ledger adds the same
+        # block partials in the same order and the ledgers are equal, not just close
+        ref = run_hip_v1_parity(inp, duration_s=90.0, snapshot_interval_s=30.0, compensated=True,
+                                block_size=BATCH_BLOCK_SIZE)
         assert np.array_equal(got["snapshot_times_s"], ref["snapshot_times_s"])
         assert np.array_equal(got["depth_snapshots_m"], ref["depth_snapshots_m"])
         assert np.array_equal(got["drain_node_volume_snapshots_m3"], ref["drain_node_volume_snapshots_m3"])

@@ -205,7 +205,8 @@ def test_uneven_batch_of_48_members_equals_plain_runs():
 def test_shared_mesh_batch_of_16_members_equals_plain_runs_and_fp64(production):
     members = synth.make_shared_mesh_ensemble(production, n_members=16)
     steps = 48
-    opts = dict(compensated=True, use_graph=True, edges_per_thread=2)
+    # one launch shape for the batch and the plain runs, so the ledger partials line up
+    opts = dict(compensated=True, use_graph=True, edges_per_thread=2, block_size=256)
     with HipEnsemble(members, **opts) as ens:
         ens.advance_to(3600.0, 3600.0, max_steps=steps, raise_on_budget=False)
         states, clocks = ens.member_states(), ens.member_clocks()
